@@ -1,0 +1,122 @@
+/*
+ * texpresso_hip.h -- C ABI of the MI355X (gfx950) BC1-BC5 encoder.
+ *
+ * The entry points mirror texpresso 2.0.2's public Rust API one to one
+ * (lib/src/lib.rs); every declaration cites the reference item it replaces.
+ * All compute runs in hand-written HIP kernels; there is no CPU fallback: an
+ * entry point that needs the GPU returns TX_ERR_NO_DEVICE when none is
+ * usable.  Output bytes are identical to the reference encoder's.
+ *
+ * Ownership: the caller owns every buffer; nothing is retained after a call
+ * returns, except the library's per-device context (streams, scratch pools)
+ * and the opaque batch/server handles, which the caller destroys.
+ * Thread safety: every entry point may be called from any thread; calls on
+ * one device are serialised on that device's context lock.
+ * Errors: negative return codes (see TX_ERR_*).  Where the reference panics
+ * (short input, short output, zero width) these return an error instead and
+ * leave the output unspecified.
+ */
+#ifndef TEXPRESSO_HIP_H
+#define TEXPRESSO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Format, lib/src/lib.rs:39-46 (same order, same meaning). */
+typedef enum { TX_BC1 = 0, TX_BC2 = 1, TX_BC3 = 2, TX_BC4 = 3, TX_BC5 = 4 } tx_format;
+
+/* Algorithm, lib/src/lib.rs:49-59; default ClusterFit (lib/src/lib.rs:61-65). */
+typedef enum { TX_RANGE_FIT = 0, TX_CLUSTER_FIT = 1, TX_ITERATIVE_CLUSTER_FIT = 2 } tx_algorithm;
+
+/* Params, lib/src/lib.rs:76-90. */
+typedef struct {
+    int32_t algorithm;             /* tx_algorithm */
+    float weights[3];              /* ColourWeights: RGB channel weights */
+    uint8_t weigh_colour_by_alpha; /* bool */
+} tx_params;
+
+/* Error codes (the reference panics in these cases). */
+#define TX_OK 0
+#define TX_ERR_FORMAT (-1)         /* format or algorithm out of range */
+#define TX_ERR_ZERO_WIDTH (-2)     /* width == 0: lib/src/lib.rs:301 chunks_mut(0) panics */
+#define TX_ERR_INPUT_SHORT (-3)    /* input slice shorter than the image: lib/src/lib.rs:324 */
+#define TX_ERR_OUTPUT_SHORT (-4)   /* output smaller than compressed_size: lib/src/lib.rs:295 */
+#define TX_ERR_OUTPUT_PARTIAL (-5) /* output ends inside a block: lib/src/lib.rs:198-214 slice panics */
+#define TX_ERR_ALIGN (-6)          /* device pointer not 4-byte aligned */
+#define TX_ERR_HIP (-7)            /* HIP runtime error */
+#define TX_ERR_NO_DEVICE (-8)      /* no usable gfx950 device */
+#define TX_ERR_BUSY (-9)           /* serving queue full / admission refused */
+#define TX_ERR_INVALID (-10)       /* bad handle or argument */
+#define TX_ERR_NOMEM (-11)         /* device memory budget exceeded */
+
+/* Params::default(), lib/src/lib.rs:92-100: ClusterFit, perceptual weights
+ * [0.2126, 0.7152, 0.0722] (lib/src/lib.rs:74), no alpha weighting. */
+tx_params tx_params_default(void);
+
+/* num_blocks, lib/src/lib.rs:103-105. */
+size_t tx_num_blocks(size_t size);
+
+/* Format::block_size, lib/src/lib.rs:159-168 (0 for an invalid format). */
+size_t tx_block_size(int format);
+
+/* Format::compressed_size, lib/src/lib.rs:175-179 (0 for an invalid format). */
+size_t tx_compressed_size(int format, size_t width, size_t height);
+
+/* Format::compress, lib/src/lib.rs:287-335.  Host buffers; synchronous.
+ * Like the reference, every whole block of `output` is written: blocks past
+ * the image (an oversized buffer) are encoded as fully masked blocks
+ * (lib/src/lib.rs:300-305).  out_len must be a multiple of the block size. */
+int tx_compress(int format, const uint8_t* rgba, size_t rgba_len, size_t width, size_t height,
+                const tx_params* params, uint8_t* output, size_t out_len);
+
+/* Format::decompress, lib/src/lib.rs:124-156.  Host buffers; synchronous.
+ * Writes the width*height*4 RGBA8 pixels. */
+int tx_decompress(int format, const uint8_t* data, size_t data_len, size_t width, size_t height, uint8_t* output,
+                  size_t out_len);
+
+/* Format::compress_block_masked, lib/src/lib.rs:188-234.  `rgba` holds 16
+ * RGBA8 pixels in row-major 4x4 order; bit i of `mask` enables pixel i. */
+int tx_compress_block_masked(int format, const uint8_t rgba[64], uint32_t mask, const tx_params* params,
+                             uint8_t* output, size_t out_len);
+
+/* Many Format::compress_block_masked calls in one launch (lib/src/lib.rs:188-234
+ * applied to each block): block b is the 64 bytes at blocks + 64*b, its mask
+ * is masks[b], its encoding goes to output + b*block_size.  Host buffers;
+ * synchronous.  No reference counterpart as a single call; it is how a
+ * caller with its own tiling reaches the per-block encoder without one
+ * launch per block. */
+int tx_compress_blocks(int format, const uint8_t* blocks, const uint32_t* masks, size_t nblocks,
+                       const tx_params* params, uint8_t* output);
+
+/* Format::decompress_block, lib/src/lib.rs:240-277. */
+int tx_decompress_block(int format, const uint8_t* block, size_t block_len, uint8_t rgba_out[64]);
+
+/* Device-resident variants of Format::compress / Format::decompress: all
+ * pointers are device memory (4-byte aligned), `stream` is a hipStream_t
+ * (NULL = the library's stream for the current device).  The call enqueues
+ * the kernel and returns without synchronising. */
+int tx_compress_device(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
+                       uint8_t* d_output, size_t out_len, void* stream);
+int tx_decompress_device(int format, const uint8_t* d_data, size_t width, size_t height, uint8_t* d_output,
+                         void* stream);
+
+/* Host copy of the four single-colour lookup tables compiled into the device
+ * code (lib/src/colourfit/single_lut.rs), 6 bytes per entry
+ * (start0,end0,err0,start1,end1,err1), order 5_3, 6_3, 5_4, 6_4. */
+void tx_single_lut(uint8_t out[6144]);
+
+/* Human-readable text for an error code. */
+const char* tx_error_string(int code);
+
+/* Name of the GPU architecture the library drives ("gfx950"), or NULL when
+ * no device is usable. */
+const char* tx_device_arch(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -444,7 +444,24 @@ typedef struct {
     v4 best_error;
     uint8_t best[8];
     int searched3, searched4; /* orderings searched (statistics only) */
+    ora_cluster_trace* trace;  /* optional: records each fit's winner */
 } cluster_fit;
+
+/* records the winner of one fit (3 or 4 colours) into the trace */
+static void trace_fit(cluster_fit* f, int nc, int it, int i, int j, int k, float err, v4 ua, v4 ub, v4 a, v4 b)
+{
+    if (!f->trace) return;
+    ora_fit_trace* t = nc == 3 ? &f->trace->fit3 : &f->trace->fit4;
+    t->won = 1;
+    t->iteration = it;
+    t->i = i; t->j = j; t->k = k;
+    t->error = err;
+    t->a_raw[0] = ua.x; t->a_raw[1] = ua.y; t->a_raw[2] = ua.z;
+    t->b_raw[0] = ub.x; t->b_raw[1] = ub.y; t->b_raw[2] = ub.z;
+    t->a[0] = a.x; t->a[1] = a.y; t->a[2] = a.z;
+    t->b[0] = b.x; t->b[1] = b.y; t->b[2] = b.z;
+    memcpy(t->order, f->order[it], 16);
+}
 
 static void cluster_init(cluster_fit* f, const colourset* s, int fmt, const float w[3], int iterate)
 {
@@ -507,7 +524,7 @@ static void cluster_compress3(cluster_fit* f)
     const v4 grid = mk4(31.0f, 63.0f, 31.0f, 0.0f);
     const v4 gridrcp = mk4(1.0f / 31.0f, 1.0f / 63.0f, 1.0f / 31.0f, 0.0f);
 
-    v4 best_start = zero, best_end = zero, best_error = f->best_error;
+    v4 best_start = zero, best_end = zero, best_error = f->best_error, best_ua = zero, best_ub = zero;
     int best_iteration = 0, best_i = 0, best_j = 0;
     v3 axis = f->principle;
 
@@ -528,6 +545,7 @@ static void cluster_compress3(cluster_fit* f)
                 v4 factor = rcp4(sub4(mul4(alpha2, beta2), mul4(alphabeta, alphabeta)));
                 v4 a = mul4(sub4(mul4(alphax, beta2), mul4(betax, alphabeta)), factor);
                 v4 b = mul4(sub4(mul4(betax, alpha2), mul4(alphax, alphabeta)), factor);
+                v4 ua = a, ub = b;
                 a = min4(one, max4(zero, a));
                 b = min4(one, max4(zero, b));
                 a = mul4(trunc4(add4(mul4(grid, a), half)), gridrcp);
@@ -541,6 +559,7 @@ static void cluster_compress3(cluster_fit* f)
                 if (any_less4(error, best_error)) {
                     best_start = a; best_end = b; best_i = i; best_j = j;
                     best_error = error; best_iteration = it;
+                    best_ua = ua; best_ub = ub;
                 }
                 if (j < count) part1 = add4(part1, f->pw[j]);
             }
@@ -558,6 +577,7 @@ static void cluster_compress3(cluster_fit* f)
         remap_indices(f->set, unordered, idx);
         write3(mk3(best_start.x, best_start.y, best_start.z), mk3(best_end.x, best_end.y, best_end.z), idx, f->best);
         f->best_error = best_error;
+        trace_fit(f, 3, best_iteration, best_i, best_j, 0, best_error.x, best_ua, best_ub, best_start, best_end);
     }
 }
 
@@ -571,7 +591,7 @@ static void cluster_compress4(cluster_fit* f)
     const v4 grid = mk4(31.0f, 63.0f, 31.0f, 0.0f);
     const v4 gridrcp = mk4(1.0f / 31.0f, 1.0f / 63.0f, 1.0f / 31.0f, 0.0f);
 
-    v4 best_start = zero, best_end = zero, best_error = f->best_error;
+    v4 best_start = zero, best_end = zero, best_error = f->best_error, best_ua = zero, best_ub = zero;
     int best_iteration = 0, best_i = 0, best_j = 0, best_k = 0;
     v3 axis = f->principle;
 
@@ -594,6 +614,7 @@ static void cluster_compress4(cluster_fit* f)
                     v4 factor = rcp4(sub4(mul4(alpha2, beta2), mul4(alphabeta, alphabeta)));
                     v4 a = mul4(sub4(mul4(alphax, beta2), mul4(betax, alphabeta)), factor);
                     v4 b = mul4(sub4(mul4(betax, alpha2), mul4(alphax, alphabeta)), factor);
+                    v4 ua = a, ub = b;
                     a = min4(one, max4(zero, a));
                     b = min4(one, max4(zero, b));
                     a = mul4(trunc4(add4(mul4(grid, a), half)), gridrcp);
@@ -607,6 +628,7 @@ static void cluster_compress4(cluster_fit* f)
                     if (any_less4(error, best_error)) {
                         best_start = a; best_end = b; best_i = i; best_j = j; best_k = k;
                         best_error = error; best_iteration = it;
+                        best_ua = ua; best_ub = ub;
                     }
                     if (k < count) part2 = add4(part2, f->pw[k]);
                 }
@@ -627,6 +649,7 @@ static void cluster_compress4(cluster_fit* f)
         remap_indices(f->set, unordered, idx);
         write4(mk3(best_start.x, best_start.y, best_start.z), mk3(best_end.x, best_end.y, best_end.z), idx, f->best);
         f->best_error = best_error;
+        trace_fit(f, 4, best_iteration, best_i, best_j, best_k, best_error.x, best_ua, best_ub, best_start, best_end);
     }
 }
 
@@ -742,8 +765,24 @@ static void compress_bc3_alpha(const uint8_t rgba[16][4], int ch, uint32_t mask,
 /* ------------------------------------------------------------------ */
 /* lib.rs:188-234 */
 
+static int compress_block_impl(int fmt, const uint8_t rgba_flat[64], uint32_t mask, const ora_params* p,
+                               uint8_t* out, ora_block_stats* st, ora_cluster_trace* tr);
+
 int ora_compress_block_masked(int fmt, const uint8_t rgba_flat[64], uint32_t mask, const ora_params* p,
                               uint8_t* out, ora_block_stats* st)
+{
+    return compress_block_impl(fmt, rgba_flat, mask, p, out, st, NULL);
+}
+
+int ora_trace_block(int fmt, const uint8_t rgba[64], uint32_t mask, const ora_params* p, uint8_t* out,
+                    ora_cluster_trace* t)
+{
+    memset(t, 0, sizeof *t);
+    return compress_block_impl(fmt, rgba, mask, p, out, NULL, t);
+}
+
+static int compress_block_impl(int fmt, const uint8_t rgba_flat[64], uint32_t mask, const ora_params* p,
+                               uint8_t* out, ora_block_stats* st, ora_cluster_trace* tr)
 {
     if (fmt < TX_BC1 || fmt > TX_BC5 || p->algorithm < 0 || p->algorithm > 2) return -1;
     pthread_once(&g_lut_once, build_luts);
@@ -762,6 +801,17 @@ int ora_compress_block_masked(int fmt, const uint8_t rgba_flat[64], uint32_t mas
         uint8_t* cb = out + (fmt == TX_BC1 ? 0 : 8);
         int is_bc1 = fmt == TX_BC1;
         if (st) { st->count = set.count; st->transparent = set.transparent; }
+        if (tr) {
+            tr->count = set.count;
+            tr->transparent = set.transparent;
+            for (int q = 0; q < 16; ++q) {
+                tr->points[q][0] = set.points[q].x; tr->points[q][1] = set.points[q].y;
+                tr->points[q][2] = set.points[q].z; tr->weights[q] = set.weights[q];
+            }
+            weighted_covariance(set.points, set.weights, set.count, tr->cov);
+            v3 pc = principle_component(tr->cov);
+            tr->principle[0] = pc.x; tr->principle[1] = pc.y; tr->principle[2] = pc.z;
+        }
         if (set.count == 1) {
             single_fit f;
             memset(&f, 0, sizeof f);
@@ -782,6 +832,7 @@ int ora_compress_block_masked(int fmt, const uint8_t rgba_flat[64], uint32_t mas
         } else {
             cluster_fit f;
             cluster_init(&f, &set, fmt, p->weights, p->algorithm == TX_ITERATIVE_CLUSTER_FIT);
+            f.trace = tr;
             if (is_bc1) { cluster_compress3(&f); if (!set.transparent) cluster_compress4(&f); }
             else cluster_compress4(&f);
             memcpy(cb, f.best, 8);
@@ -795,6 +846,49 @@ int ora_compress_block_masked(int fmt, const uint8_t rgba_flat[64], uint32_t mas
 /* lib.rs:287-335, rows split over threads like rayon's par_chunks_mut */
 
 size_t ora_block_size(int fmt) { return (fmt == TX_BC1 || fmt == TX_BC4) ? 8 : 16; }
+
+typedef struct {
+    int fmt;
+    const uint8_t* blocks;
+    const uint32_t* masks;
+    const ora_params* p;
+    uint8_t* out;
+    size_t begin, end;
+} blocks_job;
+
+static void* blocks_thread(void* arg)
+{
+    const blocks_job* j = (const blocks_job*)arg;
+    size_t bs = ora_block_size(j->fmt);
+    for (size_t b = j->begin; b < j->end; ++b)
+        ora_compress_block_masked(j->fmt, j->blocks + 64 * b, j->masks[b], j->p, j->out + b * bs, NULL);
+    return NULL;
+}
+
+/* many independent compress_block_masked calls, split over threads */
+int ora_compress_blocks(int fmt, const uint8_t* blocks, const uint32_t* masks, size_t n, const ora_params* p,
+                        uint8_t* out, int nthreads)
+{
+    if (fmt < TX_BC1 || fmt > TX_BC5) return -1;
+    pthread_once(&g_lut_once, build_luts);
+    if (nthreads < 1) nthreads = 1;
+    if ((size_t)nthreads > n) nthreads = n ? (int)n : 1;
+    blocks_job* jobs = (blocks_job*)calloc((size_t)nthreads, sizeof *jobs);
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof *th);
+    if (!jobs || !th) { free(jobs); free(th); return -6; }
+    for (int t = 0; t < nthreads; ++t) {
+        blocks_job* j = &jobs[t];
+        j->fmt = fmt; j->blocks = blocks; j->masks = masks; j->p = p; j->out = out;
+        j->begin = n * (size_t)t / (size_t)nthreads;
+        j->end = n * (size_t)(t + 1) / (size_t)nthreads;
+    }
+    for (int t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, blocks_thread, &jobs[t]);
+    blocks_thread(&jobs[0]);
+    for (int t = 1; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(jobs);
+    free(th);
+    return 0;
+}
 size_t ora_num_blocks(size_t n) { return (n + 3) / 4; }
 
 typedef struct {

@@ -116,6 +116,28 @@ def compress(fmt, rgba, w, h, p, out_len=None, threads=1, stats=False):
     return (out, st) if stats else out
 
 
+def compress_blocks(fmt, blocks, masks, p, threads=None):
+    """Many compress_block_masked calls; blocks (n, 64) uint8, masks (n,) uint32."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
+    masks = np.ascontiguousarray(masks, dtype=np.uint32).reshape(-1)
+    out = np.zeros((blocks.shape[0], block_size(fmt)), np.uint8)
+    L = lib()
+    L.ora_compress_blocks.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.POINTER(OraParams), ctypes.c_void_p, ctypes.c_int]
+    rc = L.ora_compress_blocks(fmt, blocks.ctypes.data, masks.ctypes.data, blocks.shape[0], ctypes.byref(p),
+                               out.ctypes.data, threads or default_threads())
+    if rc != 0:
+        raise ValueError("oracle compress_blocks failed (rc=%d)" % rc)
+    return out
+
+
+def default_threads():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
 def decompress_block(fmt, block):
     b = np.ascontiguousarray(np.asarray(block, dtype=np.uint8))
     out = np.zeros(64, np.uint8)

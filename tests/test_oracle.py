@@ -1,0 +1,314 @@
+"""Pins the CPU oracle (oracle/bcn_oracle.c) before it is trusted as the checker.
+
+- reference known-answer vectors (lib/src/test_data.rs via tests/golden/kat.json)
+- single-colour tables against the reference tables and their SHA-256
+- ClusterFit internals against float64 closed forms: weighted covariance,
+  power iteration, stable ordering, weighted least-squares endpoints, the
+  error expansion and the exhaustive split minimum
+- integer alpha encoder: exhaustive constant blocks, hand-derived vectors
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KAT = json.load(open(os.path.join(GOLDEN, "kat.json")))
+LUT_SHA = "43eb9a7b5508ba027322c99c481c69f5a61cb1f6d764c235f339fde3a2cf41b5"
+
+
+@pytest.mark.parametrize("blk", KAT["blocks"], ids=[b["name"] for b in KAT["blocks"]])
+@pytest.mark.parametrize("alg", [O.CLUSTER_FIT, O.RANGE_FIT, O.ITERATIVE_CLUSTER_FIT])
+def test_compression_kat(blk, alg):
+    # lib/src/lib.rs:369-393: all algorithms, uniform weights, 4x4 image
+    out = O.compress(blk["format"], np.array(blk["decoded"], np.uint8), 4, 4, O.params(alg, O.UNIFORM))
+    assert bytes(out) == bytes(blk["encoded"])
+
+
+@pytest.mark.parametrize("blk", KAT["blocks"], ids=[b["name"] for b in KAT["blocks"]])
+def test_decompression_kat(blk):
+    out = O.decompress(blk["format"], np.array(blk["encoded"], np.uint8), 4, 4)
+    assert bytes(out) == bytes(blk["decoded"])
+
+
+def test_decompression_height_not_multiple_of_4():
+    d = KAT["decode_height6"]
+    out = O.decompress(d["format"], np.array(d["encoded"], np.uint8), d["width"], d["height"])
+    assert bytes(out) == bytes(d["pixel"]) * (d["width"] * d["height"])
+
+
+def test_storage_requirements():
+    for fmt, w, h, size in KAT["compressed_size"]:
+        assert O.compressed_size(fmt, w, h) == size
+
+
+def test_single_colour_lut_matches_reference():
+    ref = open(os.path.join(GOLDEN, "single_lut.bin"), "rb").read()
+    assert hashlib.sha256(ref).hexdigest() == LUT_SHA
+    assert O.single_lut() == ref
+
+
+# ------------------------------------------------------------ float64 pins
+
+def _trace(fmt, px, mask=0xFFFF, alg=O.CLUSTER_FIT, weights=O.PERCEPTUAL, alpha=False):
+    import ctypes
+    t = _Trace()
+    rgba = np.ascontiguousarray(px, dtype=np.uint8).reshape(64)
+    out = np.zeros(16, np.uint8)
+    p = O.params(alg, weights, alpha)
+    L = O.lib()
+    L.ora_trace_block.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(O.OraParams),
+                                  ctypes.c_void_p, ctypes.POINTER(_Trace)]
+    assert L.ora_trace_block(fmt, rgba.ctypes.data, mask, ctypes.byref(p), out.ctypes.data, ctypes.byref(t)) == 0
+    return t, out[:O.block_size(fmt)]
+
+
+import ctypes  # noqa: E402
+
+
+class _Fit(ctypes.Structure):
+    _fields_ = [("won", ctypes.c_int32), ("iteration", ctypes.c_int32), ("i", ctypes.c_int32),
+                ("j", ctypes.c_int32), ("k", ctypes.c_int32), ("error", ctypes.c_float),
+                ("a_raw", ctypes.c_float * 3), ("b_raw", ctypes.c_float * 3), ("a", ctypes.c_float * 3),
+                ("b", ctypes.c_float * 3), ("order", ctypes.c_uint8 * 16)]
+
+
+class _Trace(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_int32), ("transparent", ctypes.c_int32), ("points", (ctypes.c_float * 3) * 16),
+                ("weights", ctypes.c_float * 16), ("cov", ctypes.c_float * 6), ("principle", ctypes.c_float * 3),
+                ("fit3", _Fit), ("fit4", _Fit)]
+
+
+def _random_blocks(rng, n, ncol=16):
+    out = []
+    for _ in range(n):
+        pal = rng.integers(0, 256, size=(ncol, 3))
+        pick = rng.integers(0, ncol, size=16)
+        px = np.zeros((16, 4), np.uint8)
+        px[:, :3] = pal[pick]
+        px[:, 3] = 255
+        out.append(px)
+    return out
+
+
+def _weights_of(px):
+    """Distinct colours in first-occurrence order and their pixel counts."""
+    cols, counts = [], []
+    for p in px:
+        key = tuple(int(v) for v in p[:3])
+        if key in cols:
+            counts[cols.index(key)] += 1
+        else:
+            cols.append(key)
+            counts.append(1)
+    return np.array(cols, np.float64) / 255.0, np.array(counts, np.float64)
+
+
+def test_colour_set_and_covariance_closed_form():
+    rng = np.random.default_rng(1)
+    for px in _random_blocks(rng, 300, ncol=6):
+        t, _ = _trace(O.BC3, px)
+        pts, cnt = _weights_of(px)
+        n = len(cnt)
+        assert t.count == n
+        got_pts = np.array([[t.points[q][c] for c in range(3)] for q in range(n)])
+        # u8/255 is a single correctly rounded division
+        assert np.array_equal(got_pts, (pts * 255.0).astype(np.float32) / np.float32(255.0))
+        w = np.sqrt(cnt)
+        assert np.array_equal(np.array(t.weights[:n], np.float32), w.astype(np.float32))
+        c = (pts * w[:, None]).sum(0) / w.sum()
+        d = pts - c
+        cov = (d[:, :, None] * d[:, None, :] * w[:, None, None]).sum(0)
+        want = np.array([cov[0, 0], cov[0, 1], cov[0, 2], cov[1, 1], cov[1, 2], cov[2, 2]])
+        assert np.allclose(np.array(t.cov[:]), want, rtol=1e-4, atol=1e-6)
+
+
+def test_power_iteration_closed_form():
+    # eight steps of v <- C v / max(C v) from (1,1,1), in float64
+    rng = np.random.default_rng(2)
+    for px in _random_blocks(rng, 300, ncol=8):
+        t, _ = _trace(O.BC3, px)
+        c = np.array(t.cov[:], np.float64)
+        C = np.array([[c[0], c[1], c[2]], [c[1], c[3], c[4]], [c[2], c[4], c[5]]])
+        v = np.ones(3)
+        for _ in range(8):
+            wv = C @ v
+            v = wv / wv.max()
+        assert np.allclose(np.array(t.principle[:]), v, rtol=1e-3, atol=1e-4)
+
+
+def test_nan_axis_for_equal_channel_sums():
+    # red/green checker: covariance rows sum to zero -> max(Cv) = 0 -> NaN axis
+    px = np.zeros((16, 4), np.uint8)
+    px[::2, 0] = 255
+    px[1::2, 1] = 255
+    px[:, 3] = 255
+    t, out = _trace(O.BC1, px)
+    assert t.count == 2 and all(np.isnan(t.principle[:]))
+    # padding entries (index 0, key FLT_MAX) sort before the NaN keys
+    assert list(t.fit3.order if t.fit3.won else t.fit4.order)[:14] == [0] * 14
+
+
+def _lsq_endpoints(pw_pts, pw_w, alphas, betas):
+    """Weighted least squares min sum w*(alpha*a + beta*b - x)^2 per channel."""
+    A = np.stack([alphas, betas], 1) * np.sqrt(pw_w)[:, None]
+    sol = []
+    for ch in range(3):
+        y = pw_pts[:, ch] * np.sqrt(pw_w)
+        s, *_ = np.linalg.lstsq(A, y, rcond=None)
+        sol.append(s)
+    return np.array([s[0] for s in sol]), np.array([s[1] for s in sol])
+
+
+def _split_weights(n, i, j, k, four):
+    pos = np.arange(n)
+    if four:
+        al = np.select([pos < i, pos < j, pos < k], [1.0, 2 / 3, 1 / 3], 0.0)
+    else:
+        al = np.select([pos < i, pos < j], [1.0, 0.5], 0.0)
+    return al, 1.0 - al
+
+
+def _quant(v, grid):
+    return np.trunc(np.clip(v, 0, 1) * grid + 0.5) / grid
+
+
+def _err(pts, w, al, be, a, b, cw):
+    r = al[:, None] * a[None, :] + be[:, None] * b[None, :] - pts
+    return float(((r * r * w[:, None]).sum(0) * cw).sum() - ((pts * pts * w[:, None]).sum(0) * cw).sum())
+
+
+@pytest.mark.parametrize("four", [True, False])
+def test_cluster_fit_least_squares_and_error(four):
+    rng = np.random.default_rng(3 if four else 4)
+    grid = np.array([31.0, 63.0, 31.0])
+    cw = np.array(O.PERCEPTUAL)
+    checked = 0
+    for px in _random_blocks(rng, 400, ncol=rng.integers(3, 9)):
+        fmt = O.BC3 if four else O.BC1
+        t, _ = _trace(fmt, px)
+        fit = t.fit4 if four else t.fit3
+        if not fit.won or fit.iteration != 0:
+            continue
+        pts, cnt = _weights_of(px)
+        n = len(cnt)
+        w = np.sqrt(cnt)
+        order = list(fit.order[:n])
+        P, W = pts[order], w[order]
+        al, be = _split_weights(n, fit.i, fit.j, fit.k if four else 0, four)
+        if np.linalg.matrix_rank(np.stack([al, be], 1) * np.sqrt(W)[:, None]) < 2:
+            continue
+        a, b = _lsq_endpoints(P, W, al, be)
+        assert np.allclose(np.array(fit.a_raw[:]), a, rtol=1e-3, atol=1e-4)
+        assert np.allclose(np.array(fit.b_raw[:]), b, rtol=1e-3, atol=1e-4)
+        qa, qb = _quant(a, grid), _quant(b, grid)
+        assert np.allclose(np.array(fit.a[:]), qa, atol=1e-6) and np.allclose(np.array(fit.b[:]), qb, atol=1e-6)
+        assert abs(fit.error - _err(P, W, al, be, qa, qb, cw)) < 1e-4
+        checked += 1
+    assert checked > 100
+
+
+def test_cluster_fit_split_is_exhaustive_minimum():
+    # every split of the winning ordering, evaluated in float64, is no better
+    rng = np.random.default_rng(5)
+    grid = np.array([31.0, 63.0, 31.0])
+    cw = np.array(O.PERCEPTUAL)
+    for px in _random_blocks(rng, 120, ncol=7):
+        t, _ = _trace(O.BC3, px)
+        fit = t.fit4
+        pts, cnt = _weights_of(px)
+        n = len(cnt)
+        if n < 2 or not fit.won:
+            continue
+        W = np.sqrt(cnt)[list(fit.order[:n])]
+        P = pts[list(fit.order[:n])]
+        best = np.inf
+        for i in range(n):
+            for j in range(i, n + 1):
+                for k in range(max(j, 1), n + 1):
+                    al, be = _split_weights(n, i, j, k, True)
+                    M = np.stack([al, be], 1) * np.sqrt(W)[:, None]
+                    if np.linalg.matrix_rank(M) < 2:
+                        continue
+                    a, b = _lsq_endpoints(P, W, al, be)
+                    best = min(best, _err(P, W, al, be, _quant(a, grid), _quant(b, grid), cw))
+        assert fit.error <= best + 1e-4
+
+
+def test_ordering_is_stable_sort_of_projections():
+    rng = np.random.default_rng(6)
+    for px in _random_blocks(rng, 300, ncol=5):
+        # duplicate colours in mirrored positions create projection ties
+        t, _ = _trace(O.BC3, px)
+        fit = t.fit4
+        if not fit.won or fit.iteration != 0:
+            continue
+        n = t.count
+        ax = np.array(t.principle[:], np.float32)
+        pts = np.array([[t.points[q][c] for c in range(3)] for q in range(n)], np.float32)
+        keys = (pts[:, 0] * ax[0] + pts[:, 1] * ax[1]) + pts[:, 2] * ax[2]
+        want = sorted(range(n), key=lambda q: (keys[q], q))  # Python's sort is stable
+        assert list(fit.order[:n]) == want
+
+
+# ------------------------------------------------------------ alpha encoder
+
+def test_bc4_constant_blocks_round_trip():
+    # all 256 constant blocks decode exactly (5-point mode holds the value)
+    for v in range(256):
+        px = np.zeros((16, 4), np.uint8)
+        px[:, 0] = v
+        px[:, 3] = 255
+        enc = O.compress(O.BC4, px, 4, 4, O.params())
+        dec = O.decompress(O.BC4, enc, 4, 4).reshape(16, 4)
+        assert np.all(dec[:, 0] == v), v
+
+
+def test_bc4_seven_point_mode_hand_vector():
+    # values 10,40,...,220 (no 0/255): min=10, max=220; the 7-point codebook
+    # [10,220,190,160,130,100,70,40] fits exactly (error 0) while the
+    # 5-point one does not, so the block is written in 7-point mode with the
+    # endpoints swapped to (220, 10) and indices 0<->1, x -> 9-x.
+    vals = [10, 40, 70, 100, 130, 160, 190, 220] * 2
+    px = np.zeros((16, 4), np.uint8)
+    px[:, 0] = vals
+    px[:, 3] = 255
+    enc = O.compress(O.BC4, px, 4, 4, O.params())
+    # codes7 = [10, 220, 40, 70, 100, 130, 160, 190]: fit indices
+    # 0,2,3,4,5,6,7,1 -> after the swap 1,7,6,5,4,3,2,0
+    idx = [1, 7, 6, 5, 4, 3, 2, 0] * 2
+    v0 = sum(x << (3 * q) for q, x in enumerate(idx[:8]))
+    v1 = sum(x << (3 * q) for q, x in enumerate(idx[8:]))
+    want = bytes([220, 10]) + v0.to_bytes(3, "little") + v1.to_bytes(3, "little")
+    assert bytes(enc) == want
+    dec = O.decompress(O.BC4, enc, 4, 4).reshape(16, 4)
+    assert list(dec[:, 0]) == vals
+
+
+def test_bc2_alpha_quantisation_exhaustive():
+    # q = round(a * fl(15/255)), masked -> 0; nibble 2i low, 2i+1 high
+    for a in range(256):
+        px = np.zeros((16, 4), np.uint8)
+        px[:, 3] = a
+        enc = O.compress(O.BC2, px, 4, 4, O.params())
+        q = int(np.floor(a * np.float32(15.0 / 255.0) + 0.5))
+        assert bytes(enc[:8]) == bytes([q | (q << 4)] * 8)
+
+
+def test_masked_pixels_get_zero_alpha_and_index3():
+    px = np.full((16, 4), 200, np.uint8)
+    out, _ = O.compress_block_masked(O.BC1, px, 0x0001, O.params())
+    # one valid pixel -> single colour fit; masked pixels get index 3, which
+    # write4's endpoint swap turns into 3^1 = 2 when it applies
+    idx = [(out[4 + q // 4] >> (2 * (q % 4))) & 3 for q in range(16)]
+    assert len(set(idx[1:])) == 1 and idx[1] in (2, 3) and idx[0] != idx[1]
+    # nothing valid: RangeFit with no points, write3(0, 0, all 3)
+    out, _ = O.compress_block_masked(O.BC1, px, 0, O.params())
+    assert bytes(out) == bytes([0, 0, 0, 0, 0xFF, 0xFF, 0xFF, 0xFF])
+    # BC3 alpha: min=max=0, fix_range -> (0, 5), every index 0; colour write4(0,0) -> zeros
+    out, _ = O.compress_block_masked(O.BC3, px, 0, O.params())
+    assert bytes(out) == bytes([0, 5]) + bytes(14)

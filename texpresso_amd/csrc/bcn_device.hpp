@@ -1,0 +1,725 @@
+// bcn_device.hpp -- per-block BC1-BC5 encoder for gfx950 (CDNA4).
+//
+// Execution model.  A wave64 encodes four 4x4 blocks at once: each block is
+// owned by a 16-lane group (one DPP row), lane l of the group owns pixel l
+// (l = 4*py + px, the reference's mask bit).  Per-pixel work (colour-set
+// deduplication, alpha codebook fits, index assembly and packing) runs one
+// pixel per lane; per-block sequential folds (centroid, covariance, power
+// iteration) are computed redundantly by the 16 lanes from LDS, which costs
+// the same issue slots as computing them on one lane.  The ClusterFit split
+// search -- nearly all of the arithmetic -- is spread over the 16 lanes: the
+// splits of a block are read from a candidate list and evaluated
+// independently from a prefix table S[a][b] built once per ordering, then
+// reduced with DPP row rotations to the reference's first strict minimum.
+//
+// Bit-exactness.  Every float operation is the reference's binary32 operation
+// in the reference's order (lib/src/colourfit/cluster.rs, range.rs,
+// single.rs, colourset.rs, math.rs).  The build disables FP contraction
+// (-ffp-contract=off plus the pragma below) and packed FP32 math, and keeps
+// HIP's correctly rounded division and square root.  The prefix table holds
+// exactly the values of the reference's incremental part0/part1/part2
+// accumulators: S[a][b] = ((0 + pw[a]) + pw[a+1]) + ... + pw[b-1].
+//
+// No MFMA: nothing here is a dense contraction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "tx_tables.hpp"
+
+#pragma clang fp contract(off)
+
+namespace txd {
+
+enum { BC1 = 0, BC2 = 1, BC3 = 2, BC4 = 3, BC5 = 4 };
+enum { RANGE_FIT = 0, CLUSTER_FIT = 1, ITERATIVE_CLUSTER_FIT = 2 };
+enum { KIND_SINGLE = 0, KIND_RANGE = 1, KIND_CLUSTER = 2 };
+
+struct FitParams {
+    float wr, wg, wb;      // Params::weights
+    uint32_t weigh_alpha;  // Params::weigh_colour_by_alpha
+};
+
+// Per-group LDS (one per 4x4 block in flight), 3.2 KiB.
+struct __align__(16) GroupLds {
+    float4 S[txt::kTriEntries];  // prefix sums of the weighted ordered points
+    float4 pts[16];              // distinct colours: x, y, z, sqrt(weight)
+    float4 pw[16];               // weighted points in the current order
+    uint32_t pix[16];            // the block's RGBA8 pixels
+    float keys[16];              // projections onto the ordering axis
+    uint32_t ord[8][4];          // orderings searched so far (16 bytes each)
+};
+
+extern __constant__ txt::SingleLut c_lut;
+extern __constant__ txt::CandTables c_cand;
+
+// ---------------------------------------------------------------- helpers
+
+template <int N>
+__device__ __forceinline__ uint32_t ror16(uint32_t v)
+{
+    // DPP row_ror:N -- rotate within each row of 16 lanes
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x120 + N, 0xF, 0xF, false);
+}
+template <int N>
+__device__ __forceinline__ float ror16f(float v)
+{
+    return __builtin_bit_cast(float, ror16<N>(__builtin_bit_cast(uint32_t, v)));
+}
+
+__device__ __forceinline__ uint32_t grp_or(uint32_t v)
+{
+    v |= ror16<8>(v);
+    v |= ror16<4>(v);
+    v |= ror16<2>(v);
+    v |= ror16<1>(v);
+    return v;
+}
+__device__ __forceinline__ uint32_t grp_add(uint32_t v)
+{
+    v += ror16<8>(v);
+    v += ror16<4>(v);
+    v += ror16<2>(v);
+    v += ror16<1>(v);
+    return v;
+}
+__device__ __forceinline__ uint32_t grp_min(uint32_t v)
+{
+    v = min(v, ror16<8>(v));
+    v = min(v, ror16<4>(v));
+    v = min(v, ror16<2>(v));
+    v = min(v, ror16<1>(v));
+    return v;
+}
+__device__ __forceinline__ uint32_t grp_max(uint32_t v)
+{
+    v = max(v, ror16<8>(v));
+    v = max(v, ror16<4>(v));
+    v = max(v, ror16<2>(v));
+    v = max(v, ror16<1>(v));
+    return v;
+}
+
+// (error, code) argmin over the group; ties go to the smaller code, whose
+// high bits are the split's loop-order rank.
+template <int N>
+__device__ __forceinline__ void argmin_step(float& e, uint32_t& c)
+{
+    const float e2 = ror16f<N>(e);
+    const uint32_t c2 = ror16<N>(c);
+    const bool take = (e2 < e) || (e2 == e && c2 < c);
+    e = take ? e2 : e;
+    c = take ? c2 : c;
+}
+__device__ __forceinline__ void grp_argmin(float& e, uint32_t& c)
+{
+    argmin_step<8>(e, c);
+    argmin_step<4>(e, c);
+    argmin_step<2>(e, c);
+    argmin_step<1>(e, c);
+}
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
+
+// math.rs:100-102: roundf(a).max(0).min(limit) as i32, NaN -> 0
+__device__ __forceinline__ int f32_to_i32_clamped(float a, int limit)
+{
+    return (int)fminf(fmaxf(roundf(a), 0.0f), (float)limit);
+}
+
+// colourblock.rs:27-34
+__device__ __forceinline__ uint32_t pack_565(float x, float y, float z)
+{
+    const uint32_t r = (uint32_t)f32_to_i32_clamped(31.0f * x, 31);
+    const uint32_t g = (uint32_t)f32_to_i32_clamped(63.0f * y, 63);
+    const uint32_t b = (uint32_t)f32_to_i32_clamped(31.0f * z, 31);
+    return (r << 11) | (g << 5) | b;
+}
+
+// colourblock.rs:58-94.  `idx` is this lane's pixel index; the group's
+// packed block comes back in (lo, hi) on every lane of the group.
+__device__ __forceinline__ void write_colour(bool four, float sx, float sy, float sz, float ex, float ey, float ez,
+                                             uint32_t idx, int l, uint32_t& lo, uint32_t& hi)
+{
+    uint32_t a = pack_565(sx, sy, sz), b = pack_565(ex, ey, ez);
+    if (!four) {
+        if (a > b) {  // write3: a <= b, swap palette slots 0 and 1
+            const uint32_t t = a; a = b; b = t;
+            idx = idx == 0 ? 1u : (idx == 1 ? 0u : idx);
+        }
+    } else {
+        if (a < b) {  // write4: a >= b, index ^ 1
+            const uint32_t t = a; a = b; b = t;
+            idx = (idx ^ 1u) & 3u;
+        } else if (a == b) {
+            idx = 0;
+        }
+    }
+    lo = a | (b << 16);
+    hi = grp_or((idx & 3u) << (2 * l));
+}
+
+// ------------------------------------------------------------ alpha paths
+
+// alpha.rs:27-51
+__device__ __forceinline__ void bc2_alpha(uint32_t px, bool valid, int l, uint32_t& lo, uint32_t& hi)
+{
+    const float af = (float)(px >> 24) * (15.0f / 255.0f);
+    uint32_t q = (uint32_t)f32_to_i32_clamped(af, 15);
+    if (!valid) q = 0;
+    lo = grp_or(l < 8 ? q << (4 * l) : 0u);
+    hi = grp_or(l >= 8 ? q << (4 * (l - 8)) : 0u);
+}
+
+// alpha.rs:70-77
+__device__ __forceinline__ void fix_range(int& mn, int& mx, int steps)
+{
+    if (mx - mn < steps) mx = min(mn + steps, 255);
+    if (mx - mn < steps) mn = max(mx - steps, 0);
+}
+
+// alpha.rs:79-118 for one pixel: first strict minimum over the 8 codes
+__device__ __forceinline__ uint32_t fit8(int v, const int (&codes)[8], uint32_t& least)
+{
+    least = 0xFFFFFFFFu;
+    uint32_t index = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int d = v - codes[j];
+        const uint32_t dd = (uint32_t)(d * d);
+        if (dd < least) {
+            least = dd;
+            index = (uint32_t)j;
+        }
+    }
+    return index;
+}
+
+// alpha.rs:187-256 (BC3 alpha, BC4, BC5 channels)
+__device__ __forceinline__ void bc3_alpha(int v, bool valid, int l, uint32_t& lo, uint32_t& hi)
+{
+    const uint32_t uv = (uint32_t)v;
+    int min7 = (int)grp_min(valid ? uv : 255u);
+    int max7 = (int)grp_max(valid ? uv : 0u);
+    int min5 = (int)grp_min(valid && v != 0 ? uv : 255u);
+    int max5 = (int)grp_max(valid && v != 255 ? uv : 0u);
+    if (min5 > max5) min5 = max5;
+    if (min7 > max7) min7 = max7;
+    fix_range(min5, max5, 5);
+    fix_range(min7, max7, 7);
+    int codes5[8], codes7[8];
+    codes5[0] = min5;
+    codes5[1] = max5;
+#pragma unroll
+    for (int i = 1; i < 5; ++i) codes5[1 + i] = ((5 - i) * min5 + i * max5) / 5;
+    codes5[6] = 0;
+    codes5[7] = 255;
+    // alpha.rs:238-239: slots 0 and 1 of the 7-point codebook hold min5/max5
+    codes7[0] = min5;
+    codes7[1] = max5;
+#pragma unroll
+    for (int i = 1; i < 7; ++i) codes7[1 + i] = ((7 - i) * min7 + i * max7) / 7;
+    uint32_t l5, l7;
+    uint32_t i5 = fit8(v, codes5, l5);
+    uint32_t i7 = fit8(v, codes7, l7);
+    if (!valid) {
+        i5 = 0; i7 = 0; l5 = 0; l7 = 0;
+    }
+    const uint32_t err5 = grp_add(l5), err7 = grp_add(l7);
+    int a0, a1;
+    uint32_t idx;
+    if (err5 <= err7) {  // write_alpha_block5, alpha.rs:146-165
+        if (min5 > max5) {
+            a0 = max5; a1 = min5;
+            idx = i5 == 0 ? 1u : i5 == 1 ? 0u : (i5 >= 2 && i5 <= 5) ? 7u - i5 : i5;
+        } else {
+            a0 = min5; a1 = max5; idx = i5;
+        }
+    } else {  // write_alpha_block7, alpha.rs:167-185
+        if (min7 < max7) {
+            a0 = max7; a1 = min7;
+            idx = i7 == 0 ? 1u : i7 == 1 ? 0u : 9u - i7;
+        } else {
+            a0 = min7; a1 = max7; idx = i7;
+        }
+    }
+    // alpha.rs:120-144: two 24-bit little-endian groups of eight 3-bit indices
+    const uint32_t A = grp_or(l < 8 ? idx << (3 * l) : 0u);
+    const uint32_t B = grp_or(l >= 8 ? idx << (3 * (l - 8)) : 0u);
+    lo = (uint32_t)a0 | ((uint32_t)a1 << 8) | ((A & 0xFFFFu) << 16);
+    hi = (A >> 16) | (B << 8);
+}
+
+// ----------------------------------------------------------- colour fits
+
+struct ColourSet {
+    int n;            // ColourSet::count
+    int remap;        // this pixel's point index, -1 when excluded
+    bool transparent; // BC1 block with alpha < 128 somewhere
+};
+
+// lib/src/math.rs:44-97: weighted covariance of the n points in LDS, then
+// eight power-iteration steps.  Folds run in point order on every lane.
+__device__ __forceinline__ void principal_axis(const GroupLds& L, int n, float& vx, float& vy, float& vz)
+{
+    float total = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        if (p < n) {
+            const float4 q = L.pts[p];
+            total = total + q.w;
+            cx = cx + q.x * q.w;
+            cy = cy + q.y * q.w;
+            cz = cz + q.z * q.w;
+        }
+    }
+    if (total > FLT_EPSILON) {
+        cx = cx / total;
+        cy = cy / total;
+        cz = cz / total;
+    }
+    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f, c4 = 0.0f, c5 = 0.0f;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        if (p < n) {
+            const float4 q = L.pts[p];
+            const float ax = q.x - cx, ay = q.y - cy, az = q.z - cz;
+            const float bx = ax * q.w, by = ay * q.w, bz = az * q.w;
+            c0 = c0 + ax * bx;
+            c1 = c1 + ax * by;
+            c2 = c2 + ax * bz;
+            c3 = c3 + ay * by;
+            c4 = c4 + ay * bz;
+            c5 = c5 + az * bz;
+        }
+    }
+    vx = 1.0f; vy = 1.0f; vz = 1.0f;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        float wx = c0 * vx, wy = c1 * vx, wz = c2 * vx;
+        wx = c1 * vy + wx; wy = c3 * vy + wy; wz = c4 * vy + wz;
+        wx = c2 * vz + wx; wy = c4 * vz + wy; wz = c5 * vz + wz;
+        const float a = fmaxf(wx, fmaxf(wy, wz));
+        const float r = 1.0f / a;
+        vx = wx * r; vy = wy * r; vz = wz * r;
+    }
+}
+
+// single.rs:58-98 for one codebook; returns the winning error
+__device__ __forceinline__ uint32_t single_endpoints(int tr, int tg, int cr, int cg, int cb, float (&s)[3],
+                                                     float (&e)[3], uint32_t& index)
+{
+    uint32_t best = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint8_t* r = c_lut.v[tr][cr] + 3 * k;
+        const uint8_t* g = c_lut.v[tg][cg] + 3 * k;
+        const uint8_t* b = c_lut.v[tr][cb] + 3 * k;
+        const uint32_t er = r[2], eg = g[2], eb = b[2];
+        const uint32_t err = er * er + eg * eg + eb * eb;
+        if (err < best) {
+            s[0] = (float)r[0] / 31.0f; s[1] = (float)g[0] / 63.0f; s[2] = (float)b[0] / 31.0f;
+            e[0] = (float)r[1] / 31.0f; e[1] = (float)g[1] / 63.0f; e[2] = (float)b[1] / 31.0f;
+            index = (uint32_t)(2 * k);
+            best = err;
+        }
+    }
+    return best;
+}
+
+// single.rs:100-164
+template <int FMT>
+__device__ __forceinline__ void single_fit(const GroupLds& L, const ColourSet& cs, int l, uint32_t& lo, uint32_t& hi)
+{
+    const float4 q = L.pts[0];
+    const int cr = f32_to_i32_clamped(q.x * 255.0f, 255);
+    const int cg = f32_to_i32_clamped(q.y * 255.0f, 255);
+    const int cb = f32_to_i32_clamped(q.z * 255.0f, 255);
+    uint32_t best = 0xFFFFFFFFu;
+    float s[3], e[3];
+    uint32_t index = 0;
+    if (FMT == BC1) {
+        const uint32_t err = single_endpoints(0, 1, cr, cg, cb, s, e, index);
+        if (err < best) {
+            write_colour(false, s[0], s[1], s[2], e[0], e[1], e[2], cs.remap < 0 ? 3u : index, l, lo, hi);
+            best = err;
+        }
+    }
+    if (FMT != BC1 || !cs.transparent) {
+        const uint32_t err = single_endpoints(2, 3, cr, cg, cb, s, e, index);
+        if (err < best) write_colour(true, s[0], s[1], s[2], e[0], e[1], e[2], cs.remap < 0 ? 3u : index, l, lo, hi);
+    }
+}
+
+// range.rs: clamp to [0,1] then snap to the 565 grid
+__device__ __forceinline__ float snap(float v, float grid, float rcp)
+{
+    v = fminf(1.0f, fmaxf(0.0f, v));
+    return truncf(grid * v + 0.5f) * rcp;
+}
+
+// range.rs:44-192
+template <int FMT>
+__device__ __forceinline__ void range_fit(const GroupLds& L, const ColourSet& cs, int l, const FitParams& fp,
+                                          float px, float py, float pz, uint32_t& lo, uint32_t& hi)
+{
+    const int n = cs.n;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+    if (n > 0) {
+        const float4 q0 = L.pts[0];
+        s0 = q0.x; s1 = q0.y; s2 = q0.z;
+        e0 = s0; e1 = s1; e2 = s2;
+        float mn = q0.x * px + q0.y * py + q0.z * pz;
+        float mx = mn;
+        for (int p = 1; p < n; ++p) {
+            const float4 q = L.pts[p];
+            const float d = q.x * px + q.y * py + q.z * pz;
+            if (d < mn) {
+                s0 = q.x; s1 = q.y; s2 = q.z; mn = d;
+            } else if (d > mx) {
+                e0 = q.x; e1 = q.y; e2 = q.z; mx = d;
+            }
+        }
+    }
+    s0 = snap(s0, 31.0f, 1.0f / 31.0f); s1 = snap(s1, 63.0f, 1.0f / 63.0f); s2 = snap(s2, 31.0f, 1.0f / 31.0f);
+    e0 = snap(e0, 31.0f, 1.0f / 31.0f); e1 = snap(e1, 63.0f, 1.0f / 63.0f); e2 = snap(e2, 31.0f, 1.0f / 31.0f);
+
+    float best = FLT_MAX;
+    for (int pass = (FMT == BC1 ? 3 : 4); pass <= 4; ++pass) {
+        if (pass == 4 && FMT == BC1 && cs.transparent) break;
+        float cx[4], cy[4], cz[4];
+        int nc;
+        cx[0] = s0; cy[0] = s1; cz[0] = s2;
+        cx[1] = e0; cy[1] = e1; cz[1] = e2;
+        if (pass == 3) {
+            cx[2] = s0 * 0.5f + e0 * 0.5f; cy[2] = s1 * 0.5f + e1 * 0.5f; cz[2] = s2 * 0.5f + e2 * 0.5f;
+            cx[3] = 0.0f; cy[3] = 0.0f; cz[3] = 0.0f;
+            nc = 3;
+        } else {
+            const float t2 = 2.0f / 3.0f, t1 = 1.0f / 3.0f;
+            cx[2] = s0 * t2 + e0 * t1; cy[2] = s1 * t2 + e1 * t1; cz[2] = s2 * t2 + e2 * t1;
+            cx[3] = s0 * t1 + e0 * t2; cy[3] = s1 * t1 + e1 * t2; cz[3] = s2 * t1 + e2 * t2;
+            nc = 4;
+        }
+        // range.rs:104-141: closest code per point, error folded in point order
+        float error = 0.0f;
+        uint32_t mine = 0;
+        for (int p = 0; p < n; ++p) {
+            const float4 q = L.pts[p];
+            float dist = FLT_MAX;
+            uint32_t idx = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j < nc) {
+                    const float dx = fp.wr * (q.x - cx[j]);
+                    const float dy = fp.wg * (q.y - cy[j]);
+                    const float dz = fp.wb * (q.z - cz[j]);
+                    const float d = dx * dx + dy * dy + dz * dz;
+                    if (d < dist) {
+                        dist = d;
+                        idx = (uint32_t)j;
+                    }
+                }
+            }
+            if (p == cs.remap) mine = idx;
+            error += dist;
+        }
+        if (error < best) {
+            write_colour(pass == 4, s0, s1, s2, e0, e1, e2, cs.remap < 0 ? 3u : mine, l, lo, hi);
+            best = error;
+        }
+    }
+}
+
+// One split of the ClusterFit search (cluster.rs:189-230 for compress3,
+// 320-364 for compress4).  Returns the weighted error; the quantised
+// endpoints come back through a and b.
+template <int NC>
+__device__ __forceinline__ float eval_split(const GroupLds& L, uint32_t code, const float4& xs, const FitParams& fp,
+                                            float (&a)[3], float (&b)[3])
+{
+    const float4 p0 = L.S[(code >> 16) & 15u];
+    const float4 p1 = L.S[code & 255u];
+    float ax[3], bx[3], a2, b2, ab;
+    if (NC == 4) {
+        const float4 p2 = L.S[(code >> 8) & 255u];
+        // part3 = xsum - part2 - part1 - part0
+        const float p3x = ((xs.x - p2.x) - p1.x) - p0.x;
+        const float p3y = ((xs.y - p2.y) - p1.y) - p0.y;
+        const float p3z = ((xs.z - p2.z) - p1.z) - p0.z;
+        const float p3w = ((xs.w - p2.w) - p1.w) - p0.w;
+        const float t13 = 1.0f / 3.0f, t23 = 2.0f / 3.0f, n19 = 1.0f / 9.0f, n49 = 4.0f / 9.0f;
+        // alphax = part2*(1/3,1/3,1/3,1/9) + (part1*(2/3,2/3,2/3,4/9) + part0)
+        ax[0] = p2.x * t13 + (p1.x * t23 + p0.x);
+        ax[1] = p2.y * t13 + (p1.y * t23 + p0.y);
+        ax[2] = p2.z * t13 + (p1.z * t23 + p0.z);
+        a2 = p2.w * n19 + (p1.w * n49 + p0.w);
+        // betax = part1*(1/3,..,1/9) + (part2*(2/3,..,4/9) + part3)
+        bx[0] = p1.x * t13 + (p2.x * t23 + p3x);
+        bx[1] = p1.y * t13 + (p2.y * t23 + p3y);
+        bx[2] = p1.z * t13 + (p2.z * t23 + p3z);
+        b2 = p1.w * n19 + (p2.w * n49 + p3w);
+        ab = (2.0f / 9.0f) * (p1.w + p2.w);
+    } else {
+        // part2 = xsum - part1 - part0; h = part1*(.5,.5,.5,.25)
+        const float p2x = (xs.x - p1.x) - p0.x;
+        const float p2y = (xs.y - p1.y) - p0.y;
+        const float p2z = (xs.z - p1.z) - p0.z;
+        const float p2w = (xs.w - p1.w) - p0.w;
+        const float hx = p1.x * 0.5f, hy = p1.y * 0.5f, hz = p1.z * 0.5f, hw = p1.w * 0.25f;
+        ax[0] = hx + p0.x; ax[1] = hy + p0.y; ax[2] = hz + p0.z; a2 = hw + p0.w;
+        bx[0] = hx + p2x; bx[1] = hy + p2y; bx[2] = hz + p2z; b2 = hw + p2w;
+        ab = hw;
+    }
+    const float factor = 1.0f / ((a2 * b2) - ab * ab);
+    const float grid[3] = {31.0f, 63.0f, 31.0f};
+    const float rcp[3] = {1.0f / 31.0f, 1.0f / 63.0f, 1.0f / 31.0f};
+    const float wts[3] = {fp.wr, fp.wg, fp.wb};
+    float e5[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float av = ((ax[c] * b2) - bx[c] * ab) * factor;
+        float bv = ((bx[c] * a2) - ax[c] * ab) * factor;
+        av = snap(av, grid[c], rcp[c]);
+        bv = snap(bv, grid[c], rcp[c]);
+        a[c] = av;
+        b[c] = bv;
+        const float e1 = (av * av) * a2 + ((bv * bv) * b2);
+        const float e2 = ((av * bv) * ab) - av * ax[c];
+        const float e3 = e2 - bv * bx[c];
+        const float e4 = 2.0f * e3 + e1;
+        e5[c] = e4 * wts[c];
+    }
+    return (e5[0] + e5[1]) + e5[2];
+}
+
+__device__ __forceinline__ void group_sync() { __syncthreads(); }
+
+// cluster.rs:78-136 (ordering) and cluster.rs:152-274 / 276-417 (search).
+// Executed by the whole wave; `act` marks the groups running ClusterFit.
+// `best_err`, `lo`, `hi` carry ClusterFit::best_error / best_compressed
+// across compress3 -> compress4.
+template <int NC, int ITERS>
+__device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, int l, bool act, float px, float py,
+                                            float pz, const FitParams& fp, float& best_err, uint32_t& lo,
+                                            uint32_t& hi)
+{
+    const int n = cs.n;
+    const float thr = best_err;
+    float lb_err = thr;
+    uint32_t lb_code = 0;
+    int lb_it = 0;
+    float sa[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};  // best_start / best_end
+    float axx = px, axy = py, axz = pz;
+    bool done = !act;
+    const uint32_t len = act ? (NC == 4 ? c_cand.len4[n] : c_cand.len3[n]) : 0u;
+    const uint32_t* tab = NC == 4 ? c_cand.t4 : c_cand.t3;
+
+    for (int it = 0; it < ITERS; ++it) {
+        if (!wave_any(!done)) break;
+        // ---- construct_ordering: stable sort of the projections
+        float key = FLT_MAX;
+        uint32_t idx = 0;
+        if (l < n) {
+            const float4 q = L.pts[l];
+            key = (q.x * axx + q.y * axy) + q.z * axz;
+            idx = (uint32_t)l;
+        }
+        if (!__builtin_isfinite(key)) key = __builtin_huge_valf();  // non-finite keys tie, after all finite
+        if (!done) L.keys[l] = key;
+        group_sync();
+        int rank = 0;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const float km = L.keys[m];
+            rank += (km < key) ? 1 : 0;
+            rank += (m < l && km == key) ? 1 : 0;
+        }
+        if (!done) reinterpret_cast<uint8_t*>(L.ord[it])[rank] = (uint8_t)idx;
+        group_sync();
+        if (!done) {
+            const uint4 cur = *reinterpret_cast<const uint4*>(L.ord[it]);
+            for (int p = 0; p < it; ++p) {
+                const uint4 o = *reinterpret_cast<const uint4*>(L.ord[p]);
+                if (o.x == cur.x && o.y == cur.y && o.z == cur.z && o.w == cur.w) done = true;
+            }
+        }
+        // ---- weighted points and the prefix table
+        if (!done && l < n) {
+            const uint32_t o = (L.ord[it][l >> 2] >> (8 * (l & 3))) & 255u;
+            const float4 q = L.pts[o];
+            L.pw[l] = make_float4(q.x * q.w, q.y * q.w, q.z * q.w, 1.0f * q.w);
+        }
+        group_sync();
+        if (!done) {
+            float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            L.S[txt::tri(l, l)] = s;
+            if (l == 0) L.S[txt::tri(16, 16)] = s;
+            for (int t = 0; t < 16; ++t) {
+                const int bb = l + 1 + t;
+                if (bb <= n) {
+                    const float4 w = L.pw[bb - 1];
+                    s.x = s.x + w.x; s.y = s.y + w.y; s.z = s.z + w.z; s.w = s.w + w.w;
+                    L.S[txt::tri(l, bb)] = s;
+                }
+            }
+        }
+        group_sync();
+        // ---- split search
+        const float4 xs = L.S[n];  // xsum_wsum = S[0][n]
+        const uint32_t mylen = done ? 0u : len;
+        float be = __builtin_huge_valf();
+        uint32_t bc = 0xFFFFFFFFu;
+        for (uint32_t base = 0; wave_any(base < mylen); base += 16) {
+            const uint32_t t = base + (uint32_t)l;
+            if (t < mylen) {
+                const uint32_t code = tab[t];
+                float a[3], b[3];
+                const float err = eval_split<NC>(L, code, xs, fp, a, b);
+                if (err < be || (err == be && code < bc)) {
+                    be = err;
+                    bc = code;
+                }
+            }
+        }
+        grp_argmin(be, bc);
+        // ---- iteration control, cluster.rs:222-249
+        if (!done) {
+            if (be < lb_err) {
+                lb_err = be;
+                lb_code = bc;
+                lb_it = it;
+                eval_split<NC>(L, bc, xs, fp, sa, sb);
+            }
+            if (lb_it != it) {
+                done = true;
+            } else {
+                axx = sb[0] - sa[0];
+                axy = sb[1] - sa[1];
+                axz = sb[2] - sa[2];
+            }
+        }
+    }
+
+    // ---- index assembly, cluster.rs:252-273 / 391-416
+    if (act && lb_err < thr) {
+        const int i = (int)((lb_code >> 16) & 15u);
+        const int j = (int)(lb_code & 255u) - txt::tri(i, 0);
+        const int k = NC == 4 ? (int)((lb_code >> 8) & 255u) - txt::tri(j, 0) : 0;
+        uint32_t idx = 3;
+        if (cs.remap >= 0) {
+            // the value a point ends with is set by the last range that
+            // writes it, i.e. by its largest position q >= i in the order
+            int qmax = -1;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int pq = (int)((L.ord[lb_it][q >> 2] >> (8 * (q & 3))) & 255u);
+                if (q >= i && q < n && pq == cs.remap) qmax = q;
+            }
+            if (qmax < 0) idx = 0;
+            else if (qmax < j) idx = 2;
+            else if (NC == 4 && qmax < k) idx = 3;
+            else idx = 1;
+        }
+        uint32_t nlo, nhi;
+        write_colour(NC == 4, sa[0], sa[1], sa[2], sb[0], sb[1], sb[2], idx, l, nlo, nhi);
+        lo = nlo;
+        hi = nhi;
+        best_err = lb_err;
+    }
+}
+
+// lib.rs:309-330: lane l of the group fetches pixel l of block (bx, by);
+// pixels outside the image are masked out.
+__device__ __forceinline__ uint32_t fetch_pixel(const uint8_t* __restrict__ rgba, uint32_t width, uint32_t height,
+                                                uint32_t bx, uint32_t by, bool live, int l, bool& valid)
+{
+    const uint32_t sx = bx * 4u + (uint32_t)(l & 3), sy = by * 4u + (uint32_t)(l >> 2);
+    valid = live && sx < width && sy < height;
+    return valid ? *reinterpret_cast<const uint32_t*>(rgba + 4 * ((size_t)sy * width + sx)) : 0u;
+}
+
+// Encodes one 4x4 block per 16-lane group (lib.rs:188-234): lane l holds
+// pixel l (`px`, RGBA8 little-endian) and its mask bit (`valid`).  Every lane
+// of the wave must call this (LDS barriers inside); groups without a block
+// pass valid = false and discard the result.  Returns the block words in
+// out[0..3] (8-byte formats use out[0..1]).
+template <int FMT, int ALG>
+__device__ __forceinline__ void encode_block(uint32_t px, bool valid, const FitParams& fp, GroupLds& L, int g, int l,
+                                             uint32_t (&out)[4])
+{
+    const int gshift = 16 * g;
+
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (FMT == BC2) bc2_alpha(px, valid, l, out[0], out[1]);
+    if (FMT == BC3) bc3_alpha((int)(px >> 24), valid, l, out[0], out[1]);
+    if (FMT == BC4 || FMT == BC5) bc3_alpha((int)(px & 255u), valid, l, out[0], out[1]);
+    if (FMT == BC5) bc3_alpha((int)((px >> 8) & 255u), valid, l, out[2], out[3]);
+    if (FMT == BC4 || FMT == BC5) return;
+
+    // ---- ColourSet::new, colourset.rs:35-112
+    const uint32_t rgb = px & 0xFFFFFFu, alpha = px >> 24;
+    const bool inc = valid && !(FMT == BC1 && alpha < 128u);
+    const uint32_t incmask = (uint32_t)(__ballot(inc) >> gshift) & 0xFFFFu;
+    ColourSet cs;
+    cs.transparent = FMT == BC1 && ((uint32_t)(__ballot(valid && alpha < 128u) >> gshift) & 0xFFFFu) != 0;
+    L.pix[l] = px;
+    group_sync();
+    uint32_t same = 0, asum = 0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const uint32_t pm = L.pix[m];
+        const bool s = ((incmask >> m) & 1u) && ((pm & 0xFFFFFFu) == rgb);
+        same |= (s ? 1u : 0u) << m;
+        asum += s ? (pm >> 24) + 1u : 0u;
+    }
+    const uint32_t lower = (1u << l) - 1u;
+    const uint32_t dup = same & lower;
+    const bool is_new = inc && dup == 0;
+    const uint32_t newmask = (uint32_t)(__ballot(is_new) >> gshift) & 0xFFFFu;
+    cs.n = __popc(newmask);
+    cs.remap = -1;
+    if (inc) cs.remap = is_new ? __popc(newmask & lower) : __popc(newmask & ((1u << (__ffs(dup) - 1)) - 1u));
+    if (is_new) {
+        // weights sum exactly (multiples of 1/256 below 16), then sqrtf
+        const float w = fp.weigh_alpha ? (float)asum / 256.0f : (float)__popc(same);
+        L.pts[cs.remap] = make_float4((float)(rgb & 255u) / 255.0f, (float)((rgb >> 8) & 255u) / 255.0f,
+                                      (float)((rgb >> 16) & 255u) / 255.0f, sqrtf(w));
+    }
+    group_sync();
+
+    // ---- fit selection, lib.rs:217-230
+    const int kind = cs.n == 1 ? KIND_SINGLE
+                     : (ALG == RANGE_FIT || cs.n == 0) ? KIND_RANGE : KIND_CLUSTER;
+    uint32_t lo = 0, hi = 0;
+    if (kind == KIND_SINGLE) single_fit<FMT>(L, cs, l, lo, hi);
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    if (kind != KIND_SINGLE) principal_axis(L, cs.n, vx, vy, vz);
+    if (kind == KIND_RANGE) range_fit<FMT>(L, cs, l, fp, vx, vy, vz, lo, hi);
+    if (ALG != RANGE_FIT) {
+        // wave-uniform region: run_cluster contains LDS barriers
+        constexpr int ITERS = ALG == ITERATIVE_CLUSTER_FIT ? 8 : 1;
+        const bool act = kind == KIND_CLUSTER;
+        if (wave_any(act)) {
+            float best_err = FLT_MAX;
+            uint32_t clo = 0, chi = 0;
+            if (FMT == BC1) run_cluster<3, ITERS>(L, cs, l, act, vx, vy, vz, fp, best_err, clo, chi);
+            run_cluster<4, ITERS>(L, cs, l, act && !(FMT == BC1 && cs.transparent), vx, vy, vz, fp, best_err, clo,
+                                  chi);
+            if (act) {
+                lo = clo;
+                hi = chi;
+            }
+        }
+    }
+    if (FMT == BC1) {
+        out[0] = lo;
+        out[1] = hi;
+    } else {
+        out[2] = lo;
+        out[3] = hi;
+    }
+}
+
+}  // namespace txd

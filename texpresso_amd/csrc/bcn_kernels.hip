@@ -1,0 +1,236 @@
+// bcn_kernels.hip -- kernels and launchers of the BC1-BC5 codec (gfx950).
+#include <hip/hip_runtime.h>
+
+#include "bcn_device.hpp"
+#include "bcn_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace txd {
+
+__constant__ txt::SingleLut c_lut = txt::make_single_lut();
+__constant__ txt::CandTables c_cand = txt::make_cand_tables();
+
+// ------------------------------------------------------------------ encode
+
+template <int FMT>
+__device__ __forceinline__ void store_block(uint8_t* out, uint64_t b, const uint32_t (&w)[4])
+{
+    if (FMT == BC1 || FMT == BC4)
+        *reinterpret_cast<uint2*>(out + b * 8u) = make_uint2(w[0], w[1]);
+    else
+        *reinterpret_cast<uint4*>(out + b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Plain per-texture launch: one wave per four consecutive output blocks.
+template <int FMT, int ALG>
+__global__ __launch_bounds__(64) void k_encode(const uint8_t* __restrict__ rgba, uint32_t width, uint32_t height,
+                                               uint32_t blocks_wide, uint64_t nblocks, uint8_t* __restrict__ out,
+                                               FitParams fp)
+{
+    __shared__ GroupLds lds[4];
+    const int tid = (int)threadIdx.x, g = tid >> 4, l = tid & 15;
+    const uint64_t b = (uint64_t)blockIdx.x * 4u + (uint64_t)g;
+    const bool live = b < nblocks;
+    const uint32_t bx = live ? (uint32_t)(b % blocks_wide) : 0u;
+    const uint32_t by = live ? (uint32_t)(b / blocks_wide) : 0u;
+    bool valid;
+    const uint32_t px = fetch_pixel(rgba, width, height, bx, by, live, l, valid);
+    uint32_t w[4];
+    encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
+    if (live && l == 0) store_block<FMT>(out, b, w);
+}
+
+// Explicit block list (Format::compress_block_masked over many blocks): block
+// b's 16 pixels are the 64 bytes at blocks + 64*b, its mask is masks[b].
+template <int FMT, int ALG>
+__global__ __launch_bounds__(64) void k_encode_blocks(const uint8_t* __restrict__ blocks,
+                                                      const uint32_t* __restrict__ masks, uint64_t nblocks,
+                                                      uint8_t* __restrict__ out, FitParams fp)
+{
+    __shared__ GroupLds lds[4];
+    const int tid = (int)threadIdx.x, g = tid >> 4, l = tid & 15;
+    const uint64_t b = (uint64_t)blockIdx.x * 4u + (uint64_t)g;
+    const bool live = b < nblocks;
+    const uint32_t mask = live ? masks[b] : 0u;
+    const bool valid = live && ((mask >> l) & 1u);
+    const uint32_t px = valid ? reinterpret_cast<const uint32_t*>(blocks + 64 * b)[l] : 0u;
+    uint32_t w[4];
+    encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
+    if (live && l == 0) store_block<FMT>(out, b, w);
+}
+
+// ------------------------------------------------------------------ decode
+
+// Format::decompress_block for the pixel (px, py) of `blk`,
+// colourblock.rs:96-169 and alpha.rs:53-68, 258-304.
+__device__ __forceinline__ uint32_t bc3_alpha_value(const uint8_t* blk, int p)
+{
+    const int a0 = blk[0], a1 = blk[1];
+    const int grp = p >> 3, j = p & 7;
+    const uint32_t v = (uint32_t)blk[2 + 3 * grp] | ((uint32_t)blk[3 + 3 * grp] << 8) |
+                       ((uint32_t)blk[4 + 3 * grp] << 16);
+    const int index = (int)((v >> (3 * j)) & 7u);
+    if (index == 0) return (uint32_t)a0;
+    if (index == 1) return (uint32_t)a1;
+    if (a0 <= a1) {
+        if (index == 6) return 0u;
+        if (index == 7) return 255u;
+        const int i = index - 1;
+        return (uint32_t)(((5 - i) * a0 + i * a1) / 5);
+    }
+    const int i = index - 1;
+    return (uint32_t)(((7 - i) * a0 + i * a1) / 7);
+}
+
+__device__ __forceinline__ uint32_t colour_value(const uint8_t* blk, bool is_bc1, int p)
+{
+    const uint32_t a = (uint32_t)blk[0] | ((uint32_t)blk[1] << 8);
+    const uint32_t b = (uint32_t)blk[2] | ((uint32_t)blk[3] << 8);
+    const int index = (blk[4 + (p >> 2)] >> (2 * (p & 3))) & 3;
+    const uint32_t ra = (a >> 11) & 31u, ga = (a >> 5) & 63u, ba = a & 31u;
+    const uint32_t rb = (b >> 11) & 31u, gb = (b >> 5) & 63u, bb = b & 31u;
+    const uint32_t c[2][3] = {{(ra << 3) | (ra >> 2), (ga << 2) | (ga >> 4), (ba << 3) | (ba >> 2)},
+                              {(rb << 3) | (rb >> 2), (gb << 2) | (gb >> 4), (bb << 3) | (bb >> 2)}};
+    if (index < 2) return c[index][0] | (c[index][1] << 8) | (c[index][2] << 16) | 0xFF000000u;
+    const bool three = is_bc1 && a <= b;
+    uint32_t r = 0;
+    for (int ch = 0; ch < 3; ++ch) {
+        const uint32_t x = c[0][ch], y = c[1][ch];
+        uint32_t v;
+        if (three) v = index == 2 ? (x + y) / 2 : 0u;
+        else v = index == 2 ? (2 * x + y) / 3 : (x + 2 * y) / 3;
+        r |= v << (8 * ch);
+    }
+    const uint32_t alpha = (three && index == 3) ? 0u : 255u;
+    return r | (alpha << 24);
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ data, uint32_t width, uint32_t height,
+                                                uint32_t blocks_wide, uint8_t* __restrict__ out)
+{
+    const uint32_t x = blockIdx.x * 64u + (threadIdx.x & 63u);
+    const uint32_t y = blockIdx.y * 4u + (threadIdx.x >> 6);
+    if (x >= width || y >= height) return;
+    constexpr uint32_t bs = (FMT == BC1 || FMT == BC4) ? 8u : 16u;
+    const uint8_t* blk = data + ((size_t)(y >> 2) * blocks_wide + (x >> 2)) * bs;
+    const int p = (int)((y & 3u) * 4u + (x & 3u));
+    uint32_t rgba;
+    if (FMT == BC1 || FMT == BC2 || FMT == BC3) {
+        rgba = colour_value(blk + (FMT == BC1 ? 0 : 8), FMT == BC1, p);
+    } else {
+        rgba = 0xFF000000u;
+    }
+    if (FMT == BC2) {
+        const uint32_t q = (blk[p >> 1] >> (4 * (p & 1))) & 15u;
+        rgba = (rgba & 0x00FFFFFFu) | ((q | (q << 4)) << 24);
+    }
+    if (FMT == BC3) rgba = (rgba & 0x00FFFFFFu) | (bc3_alpha_value(blk, p) << 24);
+    if (FMT == BC4) {
+        const uint32_t r = bc3_alpha_value(blk, p);
+        rgba = r | (r << 8) | (r << 16) | 0xFF000000u;
+    }
+    if (FMT == BC5) {
+        const uint32_t r = bc3_alpha_value(blk, p), g = bc3_alpha_value(blk + 8, p);
+        rgba = r | (g << 8) | 0xFF000000u;
+    }
+    *reinterpret_cast<uint32_t*>(out + 4 * ((size_t)y * width + x)) = rgba;
+}
+
+}  // namespace txd
+
+// ---------------------------------------------------------------- launchers
+
+namespace {
+
+template <int FMT, int ALG>
+hipError_t launch_encode_t(const txk::EncodeArgs& a, hipStream_t s)
+{
+    const uint64_t groups = (a.nblocks + 3) / 4;
+    if (groups == 0) return hipSuccess;
+    hipLaunchKernelGGL((txd::k_encode<FMT, ALG>), dim3((uint32_t)groups), dim3(64), 0, s, a.rgba, a.width, a.height,
+                       a.blocks_wide, a.nblocks, a.out, txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
+    return hipGetLastError();
+}
+
+template <int FMT>
+hipError_t launch_encode_f(const txk::EncodeArgs& a, hipStream_t s)
+{
+    switch (a.algorithm) {
+    case 0: return launch_encode_t<FMT, 0>(a, s);
+    case 1: return launch_encode_t<FMT, 1>(a, s);
+    default: return launch_encode_t<FMT, 2>(a, s);
+    }
+}
+
+template <int FMT, int ALG>
+hipError_t launch_blocks_t(const txk::BlockListArgs& a, hipStream_t s)
+{
+    const uint64_t groups = (a.nblocks + 3) / 4;
+    if (groups == 0) return hipSuccess;
+    hipLaunchKernelGGL((txd::k_encode_blocks<FMT, ALG>), dim3((uint32_t)groups), dim3(64), 0, s, a.blocks, a.masks,
+                       a.nblocks, a.out, txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
+    return hipGetLastError();
+}
+
+template <int FMT>
+hipError_t launch_blocks_f(const txk::BlockListArgs& a, hipStream_t s)
+{
+    switch (a.algorithm) {
+    case 0: return launch_blocks_t<FMT, 0>(a, s);
+    case 1: return launch_blocks_t<FMT, 1>(a, s);
+    default: return launch_blocks_t<FMT, 2>(a, s);
+    }
+}
+
+template <int FMT>
+hipError_t launch_decode_t(const uint8_t* data, uint32_t w, uint32_t h, uint8_t* out, hipStream_t s)
+{
+    if (w == 0 || h == 0) return hipSuccess;
+    const uint32_t bw = (w + 3) / 4;
+    hipLaunchKernelGGL((txd::k_decode<FMT>), dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, s, data, w, h, bw, out);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+namespace txk {
+
+hipError_t launch_encode(const EncodeArgs& a, hipStream_t s)
+{
+    switch (a.format) {
+    case 0: return launch_encode_f<0>(a, s);
+    case 1: return launch_encode_f<1>(a, s);
+    case 2: return launch_encode_f<2>(a, s);
+    case 3: return launch_encode_t<3, 0>(a, s);
+    case 4: return launch_encode_t<4, 0>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s)
+{
+    switch (a.format) {
+    case 0: return launch_blocks_f<0>(a, s);
+    case 1: return launch_blocks_f<1>(a, s);
+    case 2: return launch_blocks_f<2>(a, s);
+    case 3: return launch_blocks_t<3, 0>(a, s);
+    case 4: return launch_blocks_t<4, 0>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_decode(int format, const uint8_t* data, uint32_t w, uint32_t h, uint8_t* out, hipStream_t s)
+{
+    switch (format) {
+    case 0: return launch_decode_t<0>(data, w, h, out, s);
+    case 1: return launch_decode_t<1>(data, w, h, out, s);
+    case 2: return launch_decode_t<2>(data, w, h, out, s);
+    case 3: return launch_decode_t<3>(data, w, h, out, s);
+    case 4: return launch_decode_t<4>(data, w, h, out, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace txk

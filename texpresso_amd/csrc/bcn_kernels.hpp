@@ -1,0 +1,35 @@
+// bcn_kernels.hpp -- host-side launch interface of the BC1-BC5 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace txk {
+
+struct EncodeArgs {
+    int format;           // tx_format
+    int algorithm;        // tx_algorithm
+    const uint8_t* rgba;  // device, 4-byte aligned, width*height*4 bytes
+    uint32_t width, height;
+    uint32_t blocks_wide;
+    uint64_t nblocks;  // output blocks; blocks past the image are fully masked
+    uint8_t* out;      // device, nblocks * block_size bytes
+    float wr, wg, wb;
+    uint32_t weigh_alpha;
+};
+
+struct BlockListArgs {
+    int format;
+    int algorithm;
+    const uint8_t* blocks;  // device, 64 bytes (16 RGBA8 pixels) per block
+    const uint32_t* masks;  // device, one valid-pixel mask per block
+    uint64_t nblocks;
+    uint8_t* out;  // device, nblocks * block_size bytes
+    float wr, wg, wb;
+    uint32_t weigh_alpha;
+};
+
+hipError_t launch_encode(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s);
+hipError_t launch_decode(int format, const uint8_t* data, uint32_t w, uint32_t h, uint8_t* out, hipStream_t s);
+
+}  // namespace txk

@@ -1,0 +1,266 @@
+// tx_capi.hip -- C ABI of the MI355X BC1-BC5 encoder (include/texpresso_hip.h).
+//
+// Host-buffer entry points (tx_compress, tx_decompress, ...) stage through a
+// per-device scratch pool and run synchronously on the device's stream;
+// device-buffer entry points only enqueue.  There is no CPU code path.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string.h>
+#include <vector>
+
+#include "bcn_kernels.hpp"
+#include "texpresso_hip.h"
+#include "tx_context.hpp"
+#include "tx_tables.hpp"
+
+namespace {
+
+constexpr txt::SingleLut kHostLut = txt::make_single_lut();
+
+bool valid_format(int f) { return f >= TX_BC1 && f <= TX_BC5; }
+bool valid_params(const tx_params* p) { return p && p->algorithm >= 0 && p->algorithm <= 2; }
+
+// 4 * width * height without overflow; false when it does not fit
+bool image_bytes(size_t w, size_t h, size_t* out)
+{
+    if (w != 0 && h > SIZE_MAX / 4 / w) return false;
+    *out = 4 * w * h;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+tx_params tx_params_default(void)
+{
+    tx_params p;
+    p.algorithm = TX_CLUSTER_FIT;
+    p.weights[0] = 0.2126f;
+    p.weights[1] = 0.7152f;
+    p.weights[2] = 0.0722f;
+    p.weigh_colour_by_alpha = 0;
+    return p;
+}
+
+size_t tx_num_blocks(size_t size) { return (size + 3) / 4; }
+
+size_t tx_block_size(int format)
+{
+    if (!valid_format(format)) return 0;
+    return (format == TX_BC1 || format == TX_BC4) ? 8 : 16;
+}
+
+size_t tx_compressed_size(int format, size_t width, size_t height)
+{
+    return tx_num_blocks(width) * tx_num_blocks(height) * tx_block_size(format);
+}
+
+const char* tx_error_string(int code)
+{
+    switch (code) {
+    case TX_OK: return "ok";
+    case TX_ERR_FORMAT: return "invalid format or algorithm";
+    case TX_ERR_ZERO_WIDTH: return "width is zero";
+    case TX_ERR_INPUT_SHORT: return "input buffer shorter than the image";
+    case TX_ERR_OUTPUT_SHORT: return "output buffer smaller than the compressed size";
+    case TX_ERR_OUTPUT_PARTIAL: return "output buffer ends inside a block";
+    case TX_ERR_ALIGN: return "device pointer misaligned";
+    case TX_ERR_HIP: return "HIP runtime error";
+    case TX_ERR_NO_DEVICE: return "no usable gfx950 device";
+    case TX_ERR_BUSY: return "queue full";
+    case TX_ERR_INVALID: return "invalid argument or handle";
+    case TX_ERR_NOMEM: return "device memory budget exceeded";
+    default: return "unknown error";
+    }
+}
+
+void tx_single_lut(uint8_t out[6144]) { memcpy(out, &kHostLut, sizeof kHostLut); }
+
+const char* tx_device_arch(void)
+{
+    txc::DeviceContext* ctx = txc::current_context(nullptr);
+    return ctx ? ctx->arch : nullptr;
+}
+
+// Validation shared by the image entry points (lib.rs:295-305 semantics).
+static int check_compress(int format, size_t width, size_t height, const tx_params* params, size_t out_len,
+                          size_t* in_bytes)
+{
+    if (!valid_format(format) || !valid_params(params)) return TX_ERR_FORMAT;
+    if (width == 0) return TX_ERR_ZERO_WIDTH;
+    if (width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return TX_ERR_INVALID;
+    if (!image_bytes(width, height, in_bytes)) return TX_ERR_INVALID;
+    const size_t bs = tx_block_size(format);
+    if (out_len < tx_compressed_size(format, width, height)) return TX_ERR_OUTPUT_SHORT;
+    if (out_len % bs != 0) return TX_ERR_OUTPUT_PARTIAL;
+    return TX_OK;
+}
+
+static txk::EncodeArgs encode_args(int format, const tx_params* p, const uint8_t* d_in, size_t w, size_t h,
+                                   uint8_t* d_out, size_t out_len)
+{
+    txk::EncodeArgs a;
+    a.format = format;
+    a.algorithm = p->algorithm;
+    a.rgba = d_in;
+    a.width = (uint32_t)w;
+    a.height = (uint32_t)h;
+    a.blocks_wide = (uint32_t)tx_num_blocks(w);
+    a.nblocks = out_len / tx_block_size(format);
+    a.out = d_out;
+    a.wr = p->weights[0];
+    a.wg = p->weights[1];
+    a.wb = p->weights[2];
+    a.weigh_alpha = p->weigh_colour_by_alpha ? 1u : 0u;
+    return a;
+}
+
+int tx_compress(int format, const uint8_t* rgba, size_t rgba_len, size_t width, size_t height,
+                const tx_params* params, uint8_t* output, size_t out_len)
+{
+    size_t in_bytes = 0;
+    int rc = check_compress(format, width, height, params, out_len, &in_bytes);
+    if (rc != TX_OK) return rc;
+    if (rgba_len < in_bytes) return TX_ERR_INPUT_SHORT;
+    if (out_len == 0) return TX_OK;
+    if ((in_bytes && !rgba) || !output) return TX_ERR_INVALID;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const size_t in_cap = (in_bytes + 255) & ~(size_t)255;
+    uint8_t* d = ctx->scratch(in_cap + out_len);
+    if (!d) return TX_ERR_NOMEM;
+    uint8_t* d_in = d;
+    uint8_t* d_out = d + in_cap;
+    if (in_bytes && hipMemcpyAsync(d_in, rgba, in_bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return TX_ERR_HIP;
+    if (txk::launch_encode(encode_args(format, params, d_in, width, height, d_out, out_len), ctx->stream) !=
+        hipSuccess)
+        return TX_ERR_HIP;
+    if (hipMemcpyAsync(output, d_out, out_len, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    return TX_OK;
+}
+
+int tx_compress_device(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
+                       uint8_t* d_output, size_t out_len, void* stream)
+{
+    size_t in_bytes = 0;
+    int rc = check_compress(format, width, height, params, out_len, &in_bytes);
+    if (rc != TX_OK) return rc;
+    if (out_len == 0) return TX_OK;
+    if ((in_bytes && !d_rgba) || !d_output) return TX_ERR_INVALID;
+    if (((uintptr_t)d_rgba & 3u) || ((uintptr_t)d_output & (tx_block_size(format) - 1))) return TX_ERR_ALIGN;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (txk::launch_encode(encode_args(format, params, d_rgba, width, height, d_output, out_len), s) != hipSuccess)
+        return TX_ERR_HIP;
+    return TX_OK;
+}
+
+int tx_compress_block_masked(int format, const uint8_t rgba[64], uint32_t mask, const tx_params* params,
+                             uint8_t* output, size_t out_len)
+{
+    if (!valid_format(format) || !valid_params(params)) return TX_ERR_FORMAT;
+    if (!rgba || !output) return TX_ERR_INVALID;
+    if (out_len < tx_block_size(format)) return TX_ERR_OUTPUT_SHORT;
+    return tx_compress_blocks(format, rgba, &mask, 1, params, output);
+}
+
+int tx_compress_blocks(int format, const uint8_t* blocks, const uint32_t* masks, size_t nblocks,
+                       const tx_params* params, uint8_t* output)
+{
+    if (!valid_format(format) || !valid_params(params)) return TX_ERR_FORMAT;
+    if (nblocks == 0) return TX_OK;
+    if (!blocks || !masks || !output) return TX_ERR_INVALID;
+    if (nblocks > SIZE_MAX / 64) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const size_t bs = tx_block_size(format);
+    const size_t pix = 64 * nblocks, msk = (4 * nblocks + 255) & ~(size_t)255;
+    uint8_t* d = ctx->scratch(pix + msk + nblocks * bs);
+    if (!d) return TX_ERR_NOMEM;
+    uint8_t* d_pix = d;
+    uint32_t* d_mask = reinterpret_cast<uint32_t*>(d + pix);
+    uint8_t* d_out = d + pix + msk;
+    if (hipMemcpyAsync(d_pix, blocks, pix, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    if (hipMemcpyAsync(d_mask, masks, 4 * nblocks, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return TX_ERR_HIP;
+    txk::BlockListArgs a;
+    a.format = format;
+    a.algorithm = params->algorithm;
+    a.blocks = d_pix;
+    a.masks = d_mask;
+    a.nblocks = nblocks;
+    a.out = d_out;
+    a.wr = params->weights[0];
+    a.wg = params->weights[1];
+    a.wb = params->weights[2];
+    a.weigh_alpha = params->weigh_colour_by_alpha ? 1u : 0u;
+    if (txk::launch_encode_blocks(a, ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    if (hipMemcpyAsync(output, d_out, nblocks * bs, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        return TX_ERR_HIP;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    return TX_OK;
+}
+
+int tx_decompress(int format, const uint8_t* data, size_t data_len, size_t width, size_t height, uint8_t* output,
+                  size_t out_len)
+{
+    if (!valid_format(format)) return TX_ERR_FORMAT;
+    if (width == 0) return TX_ERR_ZERO_WIDTH;
+    if (width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return TX_ERR_INVALID;
+    size_t px_bytes = 0;
+    if (!image_bytes(width, height, &px_bytes)) return TX_ERR_INVALID;
+    const size_t need = tx_compressed_size(format, width, height);
+    if (data_len < need) return TX_ERR_INPUT_SHORT;
+    if (out_len < px_bytes) return TX_ERR_OUTPUT_SHORT;
+    if (px_bytes == 0) return TX_OK;
+    if (!data || !output) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const size_t in_cap = (need + 255) & ~(size_t)255;
+    uint8_t* d = ctx->scratch(in_cap + px_bytes);
+    if (!d) return TX_ERR_NOMEM;
+    if (hipMemcpyAsync(d, data, need, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    if (txk::launch_decode(format, d, (uint32_t)width, (uint32_t)height, d + in_cap, ctx->stream) != hipSuccess)
+        return TX_ERR_HIP;
+    if (hipMemcpyAsync(output, d + in_cap, px_bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        return TX_ERR_HIP;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    return TX_OK;
+}
+
+int tx_decompress_device(int format, const uint8_t* d_data, size_t width, size_t height, uint8_t* d_output,
+                         void* stream)
+{
+    if (!valid_format(format)) return TX_ERR_FORMAT;
+    if (width == 0) return TX_ERR_ZERO_WIDTH;
+    if (width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return TX_ERR_INVALID;
+    if (height == 0) return TX_OK;
+    if (!d_data || !d_output) return TX_ERR_INVALID;
+    if ((uintptr_t)d_output & 3u) return TX_ERR_ALIGN;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (txk::launch_decode(format, d_data, (uint32_t)width, (uint32_t)height, d_output, s) != hipSuccess)
+        return TX_ERR_HIP;
+    return TX_OK;
+}
+
+int tx_decompress_block(int format, const uint8_t* block, size_t block_len, uint8_t rgba_out[64])
+{
+    if (!valid_format(format)) return TX_ERR_FORMAT;
+    if (block_len < tx_block_size(format)) return TX_ERR_INPUT_SHORT;
+    return tx_decompress(format, block, tx_block_size(format), 4, 4, rgba_out, 64);
+}
+
+}  // extern "C"

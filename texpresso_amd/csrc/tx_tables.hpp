@@ -1,0 +1,156 @@
+// tx_tables.hpp -- compile-time tables for the BC1-BC5 encoder.
+//
+// Both tables are computed by constexpr functions, so the device copy
+// (__constant__) and the host copy are the same bytes by construction.
+//
+// 1. Single-colour lookup (replaces lib/src/colourfit/single_lut.rs:52-1086).
+//    For a channel value t in 0..255 the table gives, per palette slot s in
+//    {0, 2}, the quantised endpoints (start, end) and the distance `error`
+//    between t and the closest value that slot can reproduce.  The tables are
+//    regenerated from the BC1 endpoint expansion: every (start, end) pair is
+//    expanded to 8 bits, each palette slot records the first pair that hits a
+//    value exactly, and the remaining values are filled by repeated nearest
+//    neighbour relaxation (t+1 before t-1), adding 1 to the error per step.
+//
+// 2. ClusterFit candidate lists (the split loops of
+//    lib/src/colourfit/cluster.rs:182-236 and 309-380).  compress4 searches
+//    the splits (i, j, k) with 0 <= i < n, i <= j <= n, max(j,1) <= k <= n;
+//    compress3 the splits (i, j) with 0 <= i < n, max(i,1) <= j <= n.  A
+//    split belongs to the search for n distinct colours iff
+//    max(i+1, last index) <= n, so listing the n = 16 splits sorted by that
+//    bound (stable in loop order) makes the list for every n a prefix of
+//    length len[n].  Each entry packs the prefix-table offsets the split
+//    reads and its rank in the reference's loop order, which breaks error
+//    ties exactly like the reference's "first strict minimum wins":
+//      bits  0..7   tri(i, j)   (compress3: part1 offset; compress4: part1)
+//      bits  8..15  tri(j, k)   (compress4 only: part2 offset)
+//      bits 16..19  i           (part0 = S[0][i] lives at offset i)
+//      bits 20..29  loop-order rank
+//    tri(a, b) = a*(33-a)/2 + b indexes the upper triangle a <= b <= 16.
+#pragma once
+#include <stdint.h>
+
+namespace txt {
+
+constexpr int tri(int a, int b) { return ((a * (33 - a)) >> 1) + b; }
+constexpr int kTriEntries = 153;  // tri(16, 16) + 1
+constexpr int kCand4 = 967;       // compress4 splits for 16 colours
+constexpr int kCand3 = 151;       // compress3 splits for 16 colours
+
+struct SingleLut {
+    uint8_t v[4][256][6];  // [5_3, 6_3, 5_4, 6_4][value][start0,end0,err0,start1,end1,err1]
+};
+
+struct CandTables {
+    uint32_t t4[kCand4];
+    uint32_t t3[kCand3];
+    uint16_t len4[17];
+    uint16_t len3[17];
+};
+
+constexpr void build_lut_one(int bits, int colours, uint8_t (&out)[256][6])
+{
+    int st[3][256] = {}, en[3][256] = {}, er[3][256] = {};
+    bool ok[3][256] = {};
+    // slots 0 and 2 of the palette are the ones the encoder uses; slot 1 is
+    // tracked only to keep the iteration identical in shape
+    const int levels = 1 << bits;
+    for (int v1 = 0; v1 < levels; ++v1) {
+        for (int v2 = 0; v2 < levels; ++v2) {
+            const int a = (v1 << (8 - bits)) | (v1 >> (2 * bits - 8));
+            const int b = (v2 << (8 - bits)) | (v2 >> (2 * bits - 8));
+            const int c2 = colours == 3 ? (a + b) / 2 : (2 * a + b) / 3;
+            const int codes[3] = {a, b, c2};
+            for (int s = 0; s < 3; ++s) {
+                const int t = codes[s];
+                if (!ok[s][t]) {
+                    ok[s][t] = true;
+                    st[s][t] = v1;
+                    en[s][t] = v2;
+                    er[s][t] = 0;
+                }
+            }
+        }
+    }
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (int s = 0; s < 3; ++s) {
+            for (int t = 0; t < 256; ++t) {
+                for (int q = 0; q < 2; ++q) {
+                    const int nt = q == 0 ? t + 1 : t - 1;
+                    if (nt < 0 || nt > 255 || !ok[s][nt]) continue;
+                    if (!ok[s][t] || er[s][nt] + 1 < er[s][t]) {
+                        ok[s][t] = true;
+                        st[s][t] = st[s][nt];
+                        en[s][t] = en[s][nt];
+                        er[s][t] = er[s][nt] + 1;
+                        changed = true;
+                    }
+                }
+            }
+        }
+    }
+    for (int t = 0; t < 256; ++t) {
+        out[t][0] = (uint8_t)st[0][t];
+        out[t][1] = (uint8_t)en[0][t];
+        out[t][2] = (uint8_t)er[0][t];
+        out[t][3] = (uint8_t)st[2][t];
+        out[t][4] = (uint8_t)en[2][t];
+        out[t][5] = (uint8_t)er[2][t];
+    }
+}
+
+constexpr SingleLut make_single_lut()
+{
+    SingleLut L{};
+    build_lut_one(5, 3, L.v[0]);
+    build_lut_one(6, 3, L.v[1]);
+    build_lut_one(5, 4, L.v[2]);
+    build_lut_one(6, 4, L.v[3]);
+    return L;
+}
+
+constexpr CandTables make_cand_tables()
+{
+    CandTables T{};
+    // compress4: enumerate in loop order, then bucket by the bound m
+    {
+        uint32_t lex[kCand4] = {};
+        int bound[kCand4] = {};
+        int c = 0;
+        for (int i = 0; i < 16; ++i)
+            for (int j = i; j <= 16; ++j)
+                for (int k = (j > 1 ? j : 1); k <= 16; ++k) {
+                    lex[c] = (uint32_t)tri(i, j) | ((uint32_t)tri(j, k) << 8) | ((uint32_t)i << 16) |
+                             ((uint32_t)c << 20);
+                    bound[c] = (i + 1 > k) ? i + 1 : k;
+                    ++c;
+                }
+        int pos = 0;
+        for (int m = 0; m <= 16; ++m) {
+            for (int e = 0; e < kCand4; ++e)
+                if (bound[e] == m) T.t4[pos++] = lex[e];
+            T.len4[m] = (uint16_t)pos;
+        }
+    }
+    {
+        uint32_t lex[kCand3] = {};
+        int bound[kCand3] = {};
+        int c = 0;
+        for (int i = 0; i < 16; ++i)
+            for (int j = (i > 1 ? i : 1); j <= 16; ++j) {
+                lex[c] = (uint32_t)tri(i, j) | ((uint32_t)i << 16) | ((uint32_t)c << 20);
+                bound[c] = (i + 1 > j) ? i + 1 : j;
+                ++c;
+            }
+        int pos = 0;
+        for (int m = 0; m <= 16; ++m) {
+            for (int e = 0; e < kCand3; ++e)
+                if (bound[e] == m) T.t3[pos++] = lex[e];
+            T.len3[m] = (uint16_t)pos;
+        }
+    }
+    return T;
+}
+
+}  // namespace txt
