@@ -73,22 +73,15 @@ def cpu_model():
     return "unknown"
 
 
-def time_device(fmt, d_in, w, h, params, d_out, steps, warmup, torch):
-    stream = torch.cuda.current_stream()
-    handle = stream.cuda_stream
-    for _ in range(warmup):
-        fmt.compress_device(d_in.data_ptr(), w, h, params, d_out.data_ptr(), d_out.numel(), handle)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
+def time_device(tx, fmt, d_in, w, h, params, d_out, out_len, steps, warmup):
+    """Mean kernel milliseconds per step from HIP events recorded on the
+    library stream around `steps` launches, plus the wall time around them."""
+    if warmup:
+        tx.time_compress(fmt, d_in.ptr, w, h, params, d_out.ptr, out_len, warmup)
     t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(steps):
-        fmt.compress_device(d_in.data_ptr(), w, h, params, d_out.data_ptr(), d_out.numel(), handle)
-    stop.record(stream)
-    torch.cuda.synchronize()
+    ms = tx.time_compress(fmt, d_in.ptr, w, h, params, d_out.ptr, out_len, steps)
     wall = time.perf_counter() - t0
-    return start.elapsed_time(stop) / steps, wall
+    return ms / steps, wall
 
 
 def main():
@@ -105,8 +98,6 @@ def main():
               "and multi-GPU runs are not part of this build (requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     import texpresso_amd as tx
     from texpresso_amd import texgen
 
@@ -120,19 +111,20 @@ def main():
     fmt = tx.Format.Bc3
     params = tx.Params(tx.Algorithm.IterativeClusterFit)
     out_len = fmt.compressed_size(W, H)
-    d_in = torch.from_numpy(img.reshape(-1).copy()).cuda()
-    d_out = torch.zeros(out_len, dtype=torch.uint8, device="cuda")
+    d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
+    d_out = tx.DeviceBuffer(out_len)
     print("bench: BC3 IterativeClusterFit %dx%d, %d warmup + %d steps" % (W, H, args.warmup, args.steps),
           file=sys.stderr, flush=True)
-    ms, wall = time_device(fmt, d_in, W, H, params, d_out, args.steps, args.warmup, torch)
+    ms, wall = time_device(tx, fmt, d_in, W, H, params, d_out, out_len, args.steps, args.warmup)
     mpix = W * H / 1e6
     value = mpix / (ms / 1e3)
-    gpu_out = d_out.cpu().numpy()
+    gpu_out = d_out.download()
 
     # BC5 on the normal map (second half of the workload), same protocol
-    d_nm = torch.from_numpy(nmap.reshape(-1).copy()).cuda()
-    d_nm_out = torch.zeros(tx.Format.Bc5.compressed_size(W, H), dtype=torch.uint8, device="cuda")
-    ms5, _ = time_device(tx.Format.Bc5, d_nm, W, H, tx.Params(), d_nm_out, args.steps, args.warmup, torch)
+    n5 = tx.Format.Bc5.compressed_size(W, H)
+    d_nm = tx.DeviceBuffer.from_array(nmap.reshape(-1))
+    d_nm_out = tx.DeviceBuffer(n5)
+    ms5, _ = time_device(tx, tx.Format.Bc5, d_nm, W, H, tx.Params(), d_nm_out, n5, args.steps, args.warmup)
     print("bench: BC3 %.3f ms/step (%.1f Mpix/s), BC5 %.3f ms/step" % (ms, value, ms5), file=sys.stderr, flush=True)
 
     line = {

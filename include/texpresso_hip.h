@@ -109,6 +109,18 @@ int tx_decompress_device(int format, const uint8_t* d_data, size_t width, size_t
  * (start0,end0,err0,start1,end1,err1), order 5_3, 6_3, 5_4, 6_4. */
 void tx_single_lut(uint8_t out[6144]);
 
+/* Utilities (no reference counterpart): device memory on the current device
+ * and kernel timing with HIP events on the library's stream, so callers and
+ * benchmarks need no other GPU runtime binding. */
+int tx_util_malloc(void** d_ptr, size_t bytes);
+int tx_util_free(void* d_ptr);
+int tx_util_upload(void* d_dst, const void* src, size_t bytes);
+int tx_util_download(void* dst, const void* d_src, size_t bytes);
+/* Enqueues `iters` tx_compress_device launches between two events on the
+ * library stream, waits, and stores the elapsed milliseconds. */
+int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
+                          uint8_t* d_output, size_t out_len, int iters, float* ms_total);
+
 /* Human-readable text for an error code. */
 const char* tx_error_string(int code);
 

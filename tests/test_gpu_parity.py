@@ -149,22 +149,22 @@ def test_decode_parity_random_blocks(gpu, oracle, fmt):
 
 
 def test_device_entry_points_match_host_path(gpu, oracle):
-    torch = pytest.importorskip("torch")
     w, h = 100, 68
     img = texgen.with_alpha_field(texgen.rgb_texture(w, h, 8), 8)
+    d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
     for fmt in range(5):
         f = tx.Format(fmt)
         p = _params(tx.Algorithm.IterativeClusterFit)
-        d_in = torch.from_numpy(img.reshape(-1).copy()).cuda()
-        d_out = torch.zeros(f.compressed_size(w, h), dtype=torch.uint8, device="cuda")
-        s = torch.cuda.current_stream()
-        f.compress_device(d_in.data_ptr(), w, h, p, d_out.data_ptr(), d_out.numel(), s.cuda_stream)
-        d_dec = torch.zeros(4 * w * h, dtype=torch.uint8, device="cuda")
-        f.decompress_device(d_out.data_ptr(), w, h, d_dec.data_ptr(), s.cuda_stream)
-        torch.cuda.synchronize()
+        n = f.compressed_size(w, h)
+        d_out = tx.DeviceBuffer(n)
+        d_dec = tx.DeviceBuffer(4 * w * h)
+        f.compress_device(d_in.ptr, w, h, p, d_out.ptr, n)
+        f.decompress_device(d_out.ptr, w, h, d_dec.ptr)
         want = oracle.compress(fmt, img, w, h, _oparams(oracle, p))
-        assert np.array_equal(d_out.cpu().numpy(), want)
-        assert np.array_equal(d_dec.cpu().numpy(), oracle.decompress(fmt, want, w, h))
+        assert np.array_equal(d_out.download(), want)
+        assert np.array_equal(d_dec.download(), oracle.decompress(fmt, want, w, h))
+        ms = tx.time_compress(f, d_in.ptr, w, h, p, d_out.ptr, n, 3)
+        assert ms > 0 and np.array_equal(d_out.download(), want)
 
 
 def test_repeat_runs_identical_and_tiling_independent(gpu):

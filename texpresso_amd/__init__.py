@@ -104,6 +104,12 @@ def lib():
             "tx_compress_device": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_vp]),
             "tx_decompress_device": (c_int, [c_int, c_vp, c_sz, c_sz, c_vp, c_vp]),
             "tx_single_lut": (None, [c_u8p]),
+            "tx_util_malloc": (c_int, [ctypes.POINTER(ctypes.c_void_p), c_sz]),
+            "tx_util_free": (c_int, [c_vp]),
+            "tx_util_upload": (c_int, [c_vp, c_vp, c_sz]),
+            "tx_util_download": (c_int, [c_vp, c_vp, c_sz]),
+            "tx_util_time_compress": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
+                                              ctypes.POINTER(ctypes.c_float)]),
             "tx_error_string": (ctypes.c_char_p, [c_int]),
             "tx_device_arch": (ctypes.c_char_p, []),
         }
@@ -213,6 +219,57 @@ def _is_nested(x):
         return len(x) == 16 and hasattr(x[0], "__len__")
     except TypeError:
         return False
+
+
+class DeviceBuffer:
+    """Device memory on the current GPU (tx_util_malloc); `ptr` is the address."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib().tx_util_malloc(ctypes.byref(p), nbytes), "device malloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, arr):
+        import numpy as np
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        buf.upload(arr)
+        return buf
+
+    def upload(self, arr):
+        import numpy as np
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        _check(lib().tx_util_upload(self.ptr, arr.ctypes.data, arr.nbytes), "upload")
+
+    def download(self, nbytes=None):
+        import numpy as np
+        out = np.empty(self.nbytes if nbytes is None else nbytes, np.uint8)
+        _check(lib().tx_util_download(out.ctypes.data, self.ptr, out.nbytes), "download")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().tx_util_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def time_compress(fmt, d_rgba, width, height, params, d_out, out_len, iters):
+    """Milliseconds for `iters` back-to-back device launches (HIP events)."""
+    ms = ctypes.c_float()
+    p = params._c()
+    _check(lib().tx_util_time_compress(int(fmt), d_rgba, width, height, ctypes.byref(p), d_out, out_len, iters,
+                                       ctypes.byref(ms)), "time_compress")
+    return ms.value
 
 
 def compress_blocks(fmt, blocks, masks, params):

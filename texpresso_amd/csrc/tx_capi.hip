@@ -256,6 +256,63 @@ int tx_decompress_device(int format, const uint8_t* d_data, size_t width, size_t
     return TX_OK;
 }
 
+int tx_util_malloc(void** d_ptr, size_t bytes)
+{
+    if (!d_ptr) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    if (!txc::current_context(&rc)) return rc;
+    if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess) return TX_ERR_NOMEM;
+    return TX_OK;
+}
+
+int tx_util_free(void* d_ptr)
+{
+    if (!d_ptr) return TX_OK;
+    return hipFree(d_ptr) == hipSuccess ? TX_OK : TX_ERR_HIP;
+}
+
+int tx_util_upload(void* d_dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return TX_OK;
+    if (!d_dst || !src) return TX_ERR_INVALID;
+    return hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? TX_OK : TX_ERR_HIP;
+}
+
+int tx_util_download(void* dst, const void* d_src, size_t bytes)
+{
+    if (bytes == 0) return TX_OK;
+    if (!dst || !d_src) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    return hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? TX_OK : TX_ERR_HIP;
+}
+
+int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
+                          uint8_t* d_output, size_t out_len, int iters, float* ms_total)
+{
+    if (!ms_total || iters < 1) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return TX_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return TX_ERR_HIP;
+    }
+    rc = hipEventRecord(e0, ctx->stream) == hipSuccess ? TX_OK : TX_ERR_HIP;
+    for (int i = 0; i < iters && rc == TX_OK; ++i)
+        rc = tx_compress_device(format, d_rgba, width, height, params, d_output, out_len, ctx->stream);
+    if (rc == TX_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = TX_ERR_HIP;
+    if (rc == TX_OK && hipEventSynchronize(e1) != hipSuccess) rc = TX_ERR_HIP;
+    if (rc == TX_OK && hipEventElapsedTime(ms_total, e0, e1) != hipSuccess) rc = TX_ERR_HIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int tx_decompress_block(int format, const uint8_t* block, size_t block_len, uint8_t rgba_out[64])
 {
     if (!valid_format(format)) return TX_ERR_FORMAT;
