@@ -92,7 +92,17 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import texpresso_amd as tx
+        n = args.copy_calibration << 20
+        a, b = tx.DeviceBuffer(n), tx.DeviceBuffer(n)
+        for _ in range(3):
+            tx.lib().tx_util_copy(b.ptr, a.ptr, n)
+        print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the encoder shards by blocks/textures with no collective, "
               "and multi-GPU runs are not part of this build (requested --gpus %d)" % args.gpus, file=sys.stderr)
@@ -147,6 +157,11 @@ def main():
         "wall_s": round(wall, 4),
         "secondary": {"bc5_normal_map_mpix_s": round(mpix / (ms5 / 1e3), 1), "bc5_ms_per_step": round(ms5, 4)},
     }
+
+    import ctypes
+    probe = ctypes.c_double()
+    if tx.lib().tx_util_valu_probe(ctypes.byref(probe)) == 0:
+        line["secondary"]["valu_plain_fp32_measured_tops"] = round(probe.value / 1e12, 2)
 
     if not args.no_cpu:
         from oracle import oracle as O

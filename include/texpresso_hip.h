@@ -116,10 +116,16 @@ int tx_util_malloc(void** d_ptr, size_t bytes);
 int tx_util_free(void* d_ptr);
 int tx_util_upload(void* d_dst, const void* src, size_t bytes);
 int tx_util_download(void* dst, const void* d_src, size_t bytes);
+/* Device-to-device copy on the library stream (synchronous). */
+int tx_util_copy(void* d_dst, const void* d_src, size_t bytes);
 /* Enqueues `iters` tx_compress_device launches between two events on the
  * library stream, waits, and stores the elapsed milliseconds. */
 int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
                           uint8_t* d_output, size_t out_len, int iters, float* ms_total);
+
+/* Measures the device's plain FP32 VALU rate (separate multiply and add, no
+ * FMA, no packed math) in lane-operations per second. */
+int tx_util_valu_probe(double* lane_ops_per_s);
 
 /* Human-readable text for an error code. */
 const char* tx_error_string(int code);

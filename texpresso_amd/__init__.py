@@ -108,6 +108,8 @@ def lib():
             "tx_util_free": (c_int, [c_vp]),
             "tx_util_upload": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_download": (c_int, [c_vp, c_vp, c_sz]),
+            "tx_util_copy": (c_int, [c_vp, c_vp, c_sz]),
+            "tx_util_valu_probe": (c_int, [ctypes.POINTER(ctypes.c_double)]),
             "tx_util_time_compress": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
                                               ctypes.POINTER(ctypes.c_float)]),
             "tx_error_string": (ctypes.c_char_p, [c_int]),

@@ -432,18 +432,34 @@ __device__ __forceinline__ void range_fit(const GroupLds& L, const ColourSet& cs
     }
 }
 
+// The three prefix sums a split reads: part0 = S[0][i], part1, part2.
+struct SplitSums {
+    float4 p0, p1, p2;
+};
+
+template <int NC>
+__device__ __forceinline__ SplitSums load_split(const GroupLds& L, uint32_t code)
+{
+    SplitSums s;
+    s.p0 = L.S[(code >> 16) & 15u];
+    s.p1 = L.S[code & 255u];
+    if (NC == 4) s.p2 = L.S[(code >> 8) & 255u];
+    else s.p2 = s.p1;
+    return s;
+}
+
 // One split of the ClusterFit search (cluster.rs:189-230 for compress3,
 // 320-364 for compress4).  Returns the weighted error; the quantised
 // endpoints come back through a and b.
 template <int NC>
-__device__ __forceinline__ float eval_split(const GroupLds& L, uint32_t code, const float4& xs, const FitParams& fp,
-                                            float (&a)[3], float (&b)[3])
+__device__ __forceinline__ float eval_split(const SplitSums& s, const float4& xs, const FitParams& fp, float (&a)[3],
+                                            float (&b)[3])
 {
-    const float4 p0 = L.S[(code >> 16) & 15u];
-    const float4 p1 = L.S[code & 255u];
+    const float4 p0 = s.p0;
+    const float4 p1 = s.p1;
     float ax[3], bx[3], a2, b2, ab;
     if (NC == 4) {
-        const float4 p2 = L.S[(code >> 8) & 255u];
+        const float4 p2 = s.p2;
         // part3 = xsum - part2 - part1 - part0
         const float p3x = ((xs.x - p2.x) - p1.x) - p0.x;
         const float p3y = ((xs.y - p2.y) - p1.y) - p0.y;
@@ -571,17 +587,27 @@ __device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, in
         const uint32_t mylen = done ? 0u : len;
         float be = __builtin_huge_valf();
         uint32_t bc = 0xFFFFFFFFu;
+        // software pipeline: list entries two rounds ahead, prefix sums one
+        // round ahead (an unused slot reads entry 0, which is in bounds)
+        const uint32_t t0 = (uint32_t)l;
+        uint32_t c0 = t0 < mylen ? tab[t0] : 0u;
+        uint32_t c1 = t0 + 16 < mylen ? tab[t0 + 16] : 0u;
+        SplitSums s0 = load_split<NC>(L, c0);
         for (uint32_t base = 0; wave_any(base < mylen); base += 16) {
             const uint32_t t = base + (uint32_t)l;
+            const uint32_t c2 = t + 32 < mylen ? tab[t + 32] : 0u;
+            const SplitSums s1 = load_split<NC>(L, c1);
             if (t < mylen) {
-                const uint32_t code = tab[t];
                 float a[3], b[3];
-                const float err = eval_split<NC>(L, code, xs, fp, a, b);
-                if (err < be || (err == be && code < bc)) {
+                const float err = eval_split<NC>(s0, xs, fp, a, b);
+                if (err < be || (err == be && c0 < bc)) {
                     be = err;
-                    bc = code;
+                    bc = c0;
                 }
             }
+            c0 = c1;
+            c1 = c2;
+            s0 = s1;
         }
         grp_argmin(be, bc);
         // ---- iteration control, cluster.rs:222-249
@@ -590,7 +616,7 @@ __device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, in
                 lb_err = be;
                 lb_code = bc;
                 lb_it = it;
-                eval_split<NC>(L, bc, xs, fp, sa, sb);
+                eval_split<NC>(load_split<NC>(L, bc), xs, fp, sa, sb);
             }
             if (lb_it != it) {
                 done = true;

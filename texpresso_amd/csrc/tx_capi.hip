@@ -289,6 +289,17 @@ int tx_util_download(void* dst, const void* d_src, size_t bytes)
     return hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? TX_OK : TX_ERR_HIP;
 }
 
+int tx_util_copy(void* d_dst, const void* d_src, size_t bytes)
+{
+    if (bytes == 0) return TX_OK;
+    if (!d_dst || !d_src) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    if (hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    return hipStreamSynchronize(ctx->stream) == hipSuccess ? TX_OK : TX_ERR_HIP;
+}
+
 int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
                           uint8_t* d_output, size_t out_len, int iters, float* ms_total)
 {

@@ -1,0 +1,67 @@
+// tx_probe.hip -- microbenchmark of the plain FP32 VALU rate (separate
+// multiply and add, no FMA, no packed math: the only FP32 forms the
+// bit-exact encoder can use).  Used to state the encoder's compute ceiling.
+#include <hip/hip_runtime.h>
+
+#include "texpresso_hip.h"
+#include "tx_context.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int kChains = 8;
+constexpr int kUnroll = 8;
+
+__global__ __launch_bounds__(256) void k_valu_probe(float* out, int iters, float a, float b)
+{
+    float x[kChains];
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) x[c] = (float)(threadIdx.x + c) * 1e-3f;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+#pragma unroll
+            for (int c = 0; c < kChains; ++c) x[c] = x[c] * a;
+#pragma unroll
+            for (int c = 0; c < kChains; ++c) x[c] = x[c] + b;
+        }
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) s = s + x[c];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+}  // namespace
+
+extern "C" int tx_util_valu_probe(double* lane_ops_per_s)
+{
+    if (!lane_ops_per_s) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return TX_ERR_HIP;
+    const int blocks = prop.multiProcessorCount * 8;  // 8 x 256 threads per CU
+    const int threads = 256, iters = 2048;
+    float* out = nullptr;
+    if (hipMalloc(&out, sizeof(float) * blocks * threads) != hipSuccess) return TX_ERR_NOMEM;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(threads), 0, ctx->stream, out, iters, 0.9999999f, 1e-7f);
+    (void)hipEventRecord(e0, ctx->stream);
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(threads), 0, ctx->stream, out, iters, 0.9999999f, 1e-7f);
+    (void)hipEventRecord(e1, ctx->stream);
+    rc = hipEventSynchronize(e1) == hipSuccess ? TX_OK : TX_ERR_HIP;
+    float ms = 0.0f;
+    if (rc == TX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = TX_ERR_HIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    if (rc != TX_OK || ms <= 0.0f) return rc != TX_OK ? rc : TX_ERR_HIP;
+    const double ops = (double)blocks * threads * iters * kUnroll * kChains * 2.0;
+    *lane_ops_per_s = ops / (ms * 1e-3);
+    return TX_OK;
+}
