@@ -104,6 +104,54 @@ int tx_compress_device(int format, const uint8_t* d_rgba, size_t width, size_t h
 int tx_decompress_device(int format, const uint8_t* d_data, size_t width, size_t height, uint8_t* d_output,
                          void* stream);
 
+/* ---------------------------------------------------------------------
+ * Batched encoding (no single reference counterpart: it runs many
+ * Format::compress calls, lib/src/lib.rs:287-335, as one persistent launch).
+ * Every texture (or mip level) is a tx_texture with its own format and
+ * params; all are split into chunks of consecutive blocks, ordered by
+ * estimated cost, and consumed by a persistent kernel through one device
+ * work counter.  The launch (counter reset + kernel) is captured once in a
+ * hipGraph and replayed.  Outputs are byte-identical to per-texture calls.
+ */
+typedef struct {
+    int32_t format;       /* tx_format */
+    tx_params params;
+    const uint8_t* d_rgba; /* device, width*height*4 bytes, 4-byte aligned */
+    uint8_t* d_output;     /* device, tx_compressed_size bytes, aligned to the block size */
+    uint32_t width, height;
+} tx_texture;
+
+typedef struct {
+    uint32_t chunk_blocks;     /* blocks per work-queue chunk (multiple of 4) */
+    uint32_t waves_per_group;  /* waves per workgroup: 1, 2 or 4 (4 blocks per wave) */
+    uint32_t groups_per_cu;    /* persistent workgroups per compute unit */
+    uint32_t use_graph;        /* 1: capture the launch in a hipGraph and replay it */
+} tx_batch_tuning;
+
+typedef struct tx_batch tx_batch;
+
+/* Default tuning (used when `tuning` is NULL). */
+tx_batch_tuning tx_batch_tuning_default(void);
+/* Validates the textures, builds and uploads the chunk queue. */
+int tx_batch_create(tx_batch** batch, const tx_texture* textures, size_t count, const tx_batch_tuning* tuning);
+/* Enqueues one pass over every texture on `stream` (NULL = library stream). */
+int tx_batch_run(tx_batch* batch, void* stream);
+/* Enqueues the pass and waits for it; returns the elapsed ms (HIP events). */
+int tx_batch_run_timed(tx_batch* batch, int iters, float* ms_total);
+int tx_batch_destroy(tx_batch* batch);
+/* Number of chunks and blocks in the batch. */
+int tx_batch_info(const tx_batch* batch, uint64_t* chunks, uint64_t* blocks);
+
+/* Autotuner: times every tuning of the candidate grid on these textures and
+ * returns the fastest (ms = mean per pass).  `candidates`/`ncand` may be
+ * NULL/0 for the built-in grid. */
+int tx_batch_autotune(const tx_texture* textures, size_t count, const tx_batch_tuning* candidates, size_t ncand,
+                      int iters, tx_batch_tuning* best, float* best_ms);
+
+/* Baseline for the batch path: one plain launch per texture on the library
+ * stream, `iters` passes, elapsed ms (HIP events). */
+int tx_util_time_per_texture(const tx_texture* textures, size_t count, int iters, float* ms_total);
+
 /* Host copy of the four single-colour lookup tables compiled into the device
  * code (lib/src/colourfit/single_lut.rs), 6 bytes per entry
  * (start0,end0,err0,start1,end1,err1), order 5_3, 6_3, 5_4, 6_4. */

@@ -50,6 +50,26 @@ class _TxParams(ctypes.Structure):
     ]
 
 
+class _TxTexture(ctypes.Structure):
+    _fields_ = [
+        ("format", ctypes.c_int32),
+        ("params", _TxParams),
+        ("d_rgba", ctypes.c_void_p),
+        ("d_output", ctypes.c_void_p),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+    ]
+
+
+class _TxTuning(ctypes.Structure):
+    _fields_ = [
+        ("chunk_blocks", ctypes.c_uint32),
+        ("waves_per_group", ctypes.c_uint32),
+        ("groups_per_cu", ctypes.c_uint32),
+        ("use_graph", ctypes.c_uint32),
+    ]
+
+
 class Params:
     """lib/src/lib.rs:76-100; defaults ClusterFit, perceptual weights, no alpha weighting."""
 
@@ -112,6 +132,17 @@ def lib():
             "tx_util_valu_probe": (c_int, [ctypes.POINTER(ctypes.c_double)]),
             "tx_util_time_compress": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
                                               ctypes.POINTER(ctypes.c_float)]),
+            "tx_batch_tuning_default": (_TxTuning, []),
+            "tx_batch_create": (c_int, [ctypes.POINTER(c_vp), ctypes.POINTER(_TxTexture), c_sz,
+                                        ctypes.POINTER(_TxTuning)]),
+            "tx_batch_run": (c_int, [c_vp, c_vp]),
+            "tx_batch_run_timed": (c_int, [c_vp, c_int, ctypes.POINTER(ctypes.c_float)]),
+            "tx_batch_destroy": (c_int, [c_vp]),
+            "tx_batch_info": (c_int, [c_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+            "tx_batch_autotune": (c_int, [ctypes.POINTER(_TxTexture), c_sz, ctypes.POINTER(_TxTuning), c_sz, c_int,
+                                          ctypes.POINTER(_TxTuning), ctypes.POINTER(ctypes.c_float)]),
+            "tx_util_time_per_texture": (c_int, [ctypes.POINTER(_TxTexture), c_sz, c_int,
+                                                 ctypes.POINTER(ctypes.c_float)]),
             "tx_error_string": (ctypes.c_char_p, [c_int]),
             "tx_device_arch": (ctypes.c_char_p, []),
         }
@@ -272,6 +303,132 @@ def time_compress(fmt, d_rgba, width, height, params, d_out, out_len, iters):
     _check(lib().tx_util_time_compress(int(fmt), d_rgba, width, height, ctypes.byref(p), d_out, out_len, iters,
                                        ctypes.byref(ms)), "time_compress")
     return ms.value
+
+
+class Tuning:
+    """Batch kernel tuning: blocks per queue chunk, waves (x4 blocks) per
+    workgroup, persistent workgroups per CU, hipGraph replay."""
+
+    __slots__ = ("chunk_blocks", "waves_per_group", "groups_per_cu", "use_graph")
+
+    def __init__(self, chunk_blocks=64, waves_per_group=1, groups_per_cu=12, use_graph=True):
+        self.chunk_blocks = int(chunk_blocks)
+        self.waves_per_group = int(waves_per_group)
+        self.groups_per_cu = int(groups_per_cu)
+        self.use_graph = bool(use_graph)
+
+    def _c(self):
+        return _TxTuning(self.chunk_blocks, self.waves_per_group, self.groups_per_cu, 1 if self.use_graph else 0)
+
+    @classmethod
+    def _from_c(cls, t):
+        return cls(t.chunk_blocks, t.waves_per_group, t.groups_per_cu, bool(t.use_graph))
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in self.__slots__}
+
+    def __repr__(self):
+        return "Tuning(%s)" % ", ".join("%s=%r" % kv for kv in self.as_dict().items())
+
+
+def _textures(items):
+    """items: iterable of (format, params, d_rgba_ptr, d_out_ptr, width, height)."""
+    items = list(items)
+    arr = (_TxTexture * max(1, len(items)))()
+    for i, (fmt, params, d_in, d_out, w, h) in enumerate(items):
+        arr[i].format = int(fmt)
+        arr[i].params = params._c()
+        arr[i].d_rgba = d_in
+        arr[i].d_output = d_out
+        arr[i].width = w
+        arr[i].height = h
+    return arr, len(items)
+
+
+class Batch:
+    """Many textures / mip levels encoded by one persistent launch
+    (tx_batch_*).  `textures` holds (format, params, d_rgba, d_output, w, h)
+    with device addresses (e.g. DeviceBuffer.ptr)."""
+
+    def __init__(self, textures, tuning=None):
+        self._arr, self._n = _textures(textures)
+        h = ctypes.c_void_p()
+        t = tuning._c() if tuning is not None else None
+        _check(lib().tx_batch_create(ctypes.byref(h), self._arr, self._n, ctypes.byref(t) if t else None),
+               "batch create")
+        self._h = h.value
+        self.tuning = tuning or Tuning._from_c(lib().tx_batch_tuning_default())
+
+    def run(self, stream=None):
+        _check(lib().tx_batch_run(self._h, stream), "batch run")
+
+    def run_timed(self, iters=1):
+        ms = ctypes.c_float()
+        _check(lib().tx_batch_run_timed(self._h, iters, ctypes.byref(ms)), "batch run")
+        return ms.value
+
+    def info(self):
+        c, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().tx_batch_info(self._h, ctypes.byref(c), ctypes.byref(b)), "batch info")
+        return {"chunks": c.value, "blocks": b.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def time_per_texture(textures, iters=1):
+    """Milliseconds for `iters` passes of one plain launch per texture."""
+    arr, n = _textures(textures)
+    ms = ctypes.c_float()
+    _check(lib().tx_util_time_per_texture(arr, n, iters, ctypes.byref(ms)), "per-texture timing")
+    return ms.value
+
+
+def autotune(textures, candidates=None, iters=3):
+    """Times every candidate tuning (default: the built-in grid of chunk size
+    x waves per workgroup x workgroups per CU) and returns (best Tuning, ms)."""
+    arr, n = _textures(textures)
+    best, ms = _TxTuning(), ctypes.c_float()
+    if candidates:
+        cand = (_TxTuning * len(candidates))(*[c._c() for c in candidates])
+        nc = len(candidates)
+    else:
+        cand, nc = None, 0
+    _check(lib().tx_batch_autotune(arr, n, cand, nc, iters, ctypes.byref(best), ctypes.byref(ms)), "autotune")
+    return Tuning._from_c(best), ms.value
+
+
+TUNING_TABLE = os.path.join(HERE, "tuning.json")
+
+
+def tuned(signature, textures=None, table_path=TUNING_TABLE, retune=False, iters=3):
+    """Tuning for a workload `signature` from the autotuner table; runs the
+    autotuner on `textures` (and records the result) when absent."""
+    import json
+    table = {}
+    if os.path.exists(table_path):
+        with open(table_path) as f:
+            table = json.load(f)
+    if signature in table and not retune:
+        return Tuning(**table[signature]["tuning"]), table[signature].get("ms")
+    if textures is None:
+        return Tuning(), None
+    best, ms = autotune(textures, iters=iters)
+    table[signature] = {"tuning": best.as_dict(), "ms": ms}
+    try:
+        with open(table_path, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+    return best, ms
 
 
 def compress_blocks(fmt, blocks, masks, params):

@@ -51,8 +51,10 @@ struct __align__(16) GroupLds {
     uint32_t ord[8][4];          // orderings searched so far (16 bytes each)
 };
 
-extern __constant__ txt::SingleLut c_lut;
-extern __constant__ txt::CandTables c_cand;
+// Device copies of the constexpr tables (one per translation unit: the code
+// objects are built without relocatable device code).
+static __constant__ txt::SingleLut c_lut = txt::make_single_lut();
+static __constant__ txt::CandTables c_cand = txt::make_cand_tables();
 
 // ---------------------------------------------------------------- helpers
 
@@ -510,7 +512,17 @@ __device__ __forceinline__ float eval_split(const SplitSums& s, const float4& xs
     return (e5[0] + e5[1]) + e5[2];
 }
 
-__device__ __forceinline__ void group_sync() { __syncthreads(); }
+// LDS hand-off between the lanes of one group.  Groups never share LDS and a
+// wave's LDS operations execute in order, so a wavefront-scope fence (which
+// stops the compiler from reordering LDS accesses across it) is all that is
+// needed; no workgroup barrier, so waves of one workgroup stay independent
+// (the persistent batch kernel gives them different chunks).
+__device__ __forceinline__ void group_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // cluster.rs:78-136 (ordering) and cluster.rs:152-274 / 276-417 (search).
 // Executed by the whole wave; `act` marks the groups running ClusterFit.

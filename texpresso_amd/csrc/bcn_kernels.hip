@@ -8,9 +8,6 @@
 
 namespace txd {
 
-__constant__ txt::SingleLut c_lut = txt::make_single_lut();
-__constant__ txt::CandTables c_cand = txt::make_cand_tables();
-
 // ------------------------------------------------------------------ encode
 
 template <int FMT>

@@ -1,0 +1,382 @@
+// tx_batch.hip -- persistent batch encoder: many textures / mip levels of
+// mixed formats in one launch, fed by a device work counter over cost-ordered
+// chunks, captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "bcn_device.hpp"
+#include "bcn_kernels.hpp"
+#include "texpresso_hip.h"
+#include "tx_context.hpp"
+
+#pragma clang fp contract(off)
+
+namespace txd {
+
+// One texture of a batch, as the device sees it.
+struct BatchTex {
+    const uint8_t* rgba;
+    uint8_t* out;
+    uint32_t width, height, blocks_wide, nblocks;
+    uint32_t variant;  // 3*format + algorithm (BC4/BC5: algorithm 0)
+    uint32_t pad;
+    FitParams fp;
+};
+
+template <int FMT, int ALG>
+__device__ __forceinline__ void run_chunk(const BatchTex& T, uint32_t first, uint32_t count, GroupLds& L, int g,
+                                          int l)
+{
+    for (uint32_t off = 0; off < count; off += 4) {
+        const uint32_t b = first + off + (uint32_t)g;
+        const bool live = off + (uint32_t)g < count;
+        const uint32_t bx = live ? b % T.blocks_wide : 0u;
+        const uint32_t by = live ? b / T.blocks_wide : 0u;
+        bool valid;
+        const uint32_t px = fetch_pixel(T.rgba, T.width, T.height, bx, by, live, l, valid);
+        uint32_t w[4];
+        encode_block<FMT, ALG>(px, valid, T.fp, L, g, l, w);
+        if (live && l == 0) {
+            if (FMT == BC1 || FMT == BC4)
+                *reinterpret_cast<uint2*>(T.out + (size_t)b * 8u) = make_uint2(w[0], w[1]);
+            else
+                *reinterpret_cast<uint4*>(T.out + (size_t)b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+// Persistent kernel: every wave takes chunks {texture, first block, count}
+// from `counter` until the queue is exhausted.  A chunk's texture fixes the
+// format and algorithm for the whole wave, so the dispatch is wave-uniform.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_batch(const BatchTex* __restrict__ tex,
+                                                     const uint4* __restrict__ chunks, uint32_t nchunks,
+                                                     uint32_t* __restrict__ counter)
+{
+    __shared__ GroupLds lds[4 * WAVES];
+    const int tid = (int)threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, l = lane & 15;
+    GroupLds& L = lds[4 * wave + g];
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(counter, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= nchunks) break;
+        const uint4 ch = chunks[c];
+        const BatchTex& T = tex[ch.x];
+        switch (T.variant) {
+        case 0: run_chunk<BC1, RANGE_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 1: run_chunk<BC1, CLUSTER_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 2: run_chunk<BC1, ITERATIVE_CLUSTER_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 3: run_chunk<BC2, RANGE_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 4: run_chunk<BC2, CLUSTER_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 5: run_chunk<BC2, ITERATIVE_CLUSTER_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 6: run_chunk<BC3, RANGE_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 7: run_chunk<BC3, CLUSTER_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 8: run_chunk<BC3, ITERATIVE_CLUSTER_FIT>(T, ch.y, ch.z, L, g, l); break;
+        case 9: run_chunk<BC4, RANGE_FIT>(T, ch.y, ch.z, L, g, l); break;
+        default: run_chunk<BC5, RANGE_FIT>(T, ch.y, ch.z, L, g, l); break;
+        }
+    }
+}
+
+// Clears the work counter with a kernel (a memset node is not replay-safe
+// in a graph on this platform).
+__global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
+
+}  // namespace txd
+
+struct tx_batch {
+    int device = -1;
+    txd::BatchTex* d_tex = nullptr;
+    uint4* d_chunks = nullptr;
+    uint32_t* d_counter = nullptr;
+    uint32_t nchunks = 0;
+    uint64_t nblocks = 0;
+    tx_batch_tuning tuning{};
+    uint32_t grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+bool valid_texture(const tx_texture& t)
+{
+    if (t.format < TX_BC1 || t.format > TX_BC5) return false;
+    if (t.params.algorithm < 0 || t.params.algorithm > 2) return false;
+    if (t.width == 0 || t.width > 0x7FFFFFFFu || t.height > 0x7FFFFFFFu) return false;
+    if (t.height > 0 && (!t.d_rgba || !t.d_output)) return false;
+    return true;
+}
+
+// Relative per-block cost used to order chunks (most expensive first).
+uint32_t cost_class(const tx_texture& t)
+{
+    if (t.format == TX_BC4 || t.format == TX_BC5) return 0;
+    if (t.params.algorithm == TX_RANGE_FIT) return 1;
+    if (t.params.algorithm == TX_CLUSTER_FIT) return t.format == TX_BC1 ? 3 : 2;
+    return 4;
+}
+
+hipError_t launch_batch(tx_batch* b, hipStream_t s)
+{
+    hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, b->d_counter);
+    switch (b->tuning.waves_per_group) {
+    case 4:
+        hipLaunchKernelGGL(txd::k_batch<4>, dim3(b->grid), dim3(256), 0, s, b->d_tex, b->d_chunks, b->nchunks,
+                           b->d_counter);
+        break;
+    case 2:
+        hipLaunchKernelGGL(txd::k_batch<2>, dim3(b->grid), dim3(128), 0, s, b->d_tex, b->d_chunks, b->nchunks,
+                           b->d_counter);
+        break;
+    default:
+        hipLaunchKernelGGL(txd::k_batch<1>, dim3(b->grid), dim3(64), 0, s, b->d_tex, b->d_chunks, b->nchunks,
+                           b->d_counter);
+        break;
+    }
+    return hipGetLastError();
+}
+
+void free_batch(tx_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_tex) (void)hipFree(b->d_tex);
+    if (b->d_chunks) (void)hipFree(b->d_chunks);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+tx_batch_tuning tx_batch_tuning_default(void)
+{
+    tx_batch_tuning t;
+    t.chunk_blocks = 64;
+    t.waves_per_group = 1;
+    t.groups_per_cu = 12;
+    t.use_graph = 1;
+    return t;
+}
+
+int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, const tx_batch_tuning* tuning)
+{
+    if (!out) return TX_ERR_INVALID;
+    *out = nullptr;
+    if (count && !textures) return TX_ERR_INVALID;
+    if (count > 0xFFFFFFFFu) return TX_ERR_INVALID;
+    tx_batch_tuning t = tuning ? *tuning : tx_batch_tuning_default();
+    if (t.chunk_blocks == 0 || t.chunk_blocks % 4 != 0) return TX_ERR_INVALID;
+    if (t.waves_per_group != 1 && t.waves_per_group != 2 && t.waves_per_group != 4) return TX_ERR_INVALID;
+    if (t.groups_per_cu == 0 || t.groups_per_cu > 64) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+
+    std::vector<txd::BatchTex> tex(count);
+    std::vector<uint4> chunks;
+    std::vector<uint32_t> cls;
+    uint64_t nblocks = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const tx_texture& s = textures[i];
+        if (!valid_texture(s)) return TX_ERR_INVALID;
+        const size_t bs = tx_block_size(s.format);
+        if (((uintptr_t)s.d_rgba & 3u) || ((uintptr_t)s.d_output & (bs - 1))) return TX_ERR_ALIGN;
+        txd::BatchTex& d = tex[i];
+        d.rgba = s.d_rgba;
+        d.out = s.d_output;
+        d.width = s.width;
+        d.height = s.height;
+        d.blocks_wide = (s.width + 3) / 4;
+        const uint64_t nb = (uint64_t)d.blocks_wide * ((s.height + 3) / 4);
+        if (nb > 0xFFFFFFFFull) return TX_ERR_INVALID;
+        d.nblocks = (uint32_t)nb;
+        d.variant = (s.format >= TX_BC4) ? 9u + (uint32_t)(s.format - TX_BC4)
+                                         : 3u * (uint32_t)s.format + (uint32_t)s.params.algorithm;
+        d.pad = 0;
+        d.fp = txd::FitParams{s.params.weights[0], s.params.weights[1], s.params.weights[2],
+                              s.params.weigh_colour_by_alpha ? 1u : 0u};
+        for (uint32_t f = 0; f < d.nblocks; f += t.chunk_blocks) {
+            chunks.push_back(make_uint4((uint32_t)i, f, std::min(t.chunk_blocks, d.nblocks - f), 0u));
+            cls.push_back(cost_class(s));
+        }
+        nblocks += nb;
+    }
+    // most expensive chunks first (longest-processing-time order); stable
+    // so each texture's chunks stay in block order
+    std::vector<size_t> order(chunks.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cls[a] > cls[b]; });
+    std::vector<uint4> sorted(chunks.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = chunks[order[i]];
+
+    tx_batch* b = new (std::nothrow) tx_batch();
+    if (!b) return TX_ERR_NOMEM;
+    b->device = ctx->device;
+    b->tuning = t;
+    b->nchunks = (uint32_t)sorted.size();
+    b->nblocks = nblocks;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
+        free_batch(b);
+        return TX_ERR_HIP;
+    }
+    b->grid = (uint32_t)prop.multiProcessorCount * t.groups_per_cu;
+    if (hipMalloc(&b->d_tex, sizeof(txd::BatchTex) * std::max<size_t>(1, tex.size())) != hipSuccess ||
+        hipMalloc(&b->d_chunks, sizeof(uint4) * std::max<size_t>(1, sorted.size())) != hipSuccess ||
+        hipMalloc(&b->d_counter, 256) != hipSuccess) {
+        free_batch(b);
+        return TX_ERR_NOMEM;
+    }
+    if ((tex.size() && hipMemcpy(b->d_tex, tex.data(), sizeof(txd::BatchTex) * tex.size(),
+                                 hipMemcpyHostToDevice) != hipSuccess) ||
+        (sorted.size() && hipMemcpy(b->d_chunks, sorted.data(), sizeof(uint4) * sorted.size(),
+                                    hipMemcpyHostToDevice) != hipSuccess)) {
+        free_batch(b);
+        return TX_ERR_HIP;
+    }
+    *out = b;
+    return TX_OK;
+}
+
+int tx_batch_run(tx_batch* b, void* stream)
+{
+    if (!b) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    if (ctx->device != b->device) return TX_ERR_INVALID;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (b->nchunks == 0) return TX_OK;
+    if (!b->tuning.use_graph) return launch_batch(b, s) == hipSuccess ? TX_OK : TX_ERR_HIP;
+    if (!b->exec) {
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return TX_ERR_HIP;
+        const hipError_t le = launch_batch(b, s);
+        const hipError_t ce = hipStreamEndCapture(s, &b->graph);
+        if (le != hipSuccess || ce != hipSuccess) return TX_ERR_HIP;
+        if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) return TX_ERR_HIP;
+    }
+    return hipGraphLaunch(b->exec, s) == hipSuccess ? TX_OK : TX_ERR_HIP;
+}
+
+int tx_batch_run_timed(tx_batch* b, int iters, float* ms_total)
+{
+    if (!b || !ms_total || iters < 1) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return TX_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return TX_ERR_HIP;
+    }
+    rc = hipEventRecord(e0, ctx->stream) == hipSuccess ? TX_OK : TX_ERR_HIP;
+    for (int i = 0; i < iters && rc == TX_OK; ++i) rc = tx_batch_run(b, ctx->stream);
+    if (rc == TX_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = TX_ERR_HIP;
+    if (rc == TX_OK && hipEventSynchronize(e1) != hipSuccess) rc = TX_ERR_HIP;
+    if (rc == TX_OK && hipEventElapsedTime(ms_total, e0, e1) != hipSuccess) rc = TX_ERR_HIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int tx_batch_destroy(tx_batch* b)
+{
+    if (!b) return TX_OK;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    free_batch(b);
+    return TX_OK;
+}
+
+int tx_batch_info(const tx_batch* b, uint64_t* chunks, uint64_t* blocks)
+{
+    if (!b) return TX_ERR_INVALID;
+    if (chunks) *chunks = b->nchunks;
+    if (blocks) *blocks = b->nblocks;
+    return TX_OK;
+}
+
+int tx_batch_autotune(const tx_texture* textures, size_t count, const tx_batch_tuning* candidates, size_t ncand,
+                      int iters, tx_batch_tuning* best, float* best_ms)
+{
+    if (!best || !best_ms || iters < 1) return TX_ERR_INVALID;
+    std::vector<tx_batch_tuning> grid;
+    if (candidates && ncand) {
+        grid.assign(candidates, candidates + ncand);
+    } else {
+        const uint32_t chunk[] = {16, 32, 64, 128, 256};
+        const uint32_t wpg[] = {1, 2, 4};
+        const uint32_t waves_per_cu[] = {4, 8, 12};
+        for (uint32_t c : chunk)
+            for (uint32_t w : wpg)
+                for (uint32_t v : waves_per_cu) {
+                    tx_batch_tuning t;
+                    t.chunk_blocks = c;
+                    t.waves_per_group = w;
+                    t.groups_per_cu = std::max(1u, v / w);
+                    t.use_graph = 1;
+                    grid.push_back(t);
+                }
+    }
+    float best_t = -1.0f;
+    for (const tx_batch_tuning& t : grid) {
+        tx_batch* b = nullptr;
+        int rc = tx_batch_create(&b, textures, count, &t);
+        if (rc != TX_OK) return rc;
+        float ms = 0.0f;
+        rc = tx_batch_run_timed(b, 1, &ms);  // warm-up (and graph capture)
+        if (rc == TX_OK) rc = tx_batch_run_timed(b, iters, &ms);
+        tx_batch_destroy(b);
+        if (rc != TX_OK) return rc;
+        ms /= (float)iters;
+        if (best_t < 0.0f || ms < best_t) {
+            best_t = ms;
+            *best = t;
+        }
+    }
+    *best_ms = best_t;
+    return TX_OK;
+}
+
+int tx_util_time_per_texture(const tx_texture* textures, size_t count, int iters, float* ms_total)
+{
+    if (!ms_total || iters < 1 || (count && !textures)) return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    for (size_t i = 0; i < count; ++i)
+        if (!valid_texture(textures[i])) return TX_ERR_INVALID;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return TX_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return TX_ERR_HIP;
+    }
+    rc = hipEventRecord(e0, ctx->stream) == hipSuccess ? TX_OK : TX_ERR_HIP;
+    for (int it = 0; it < iters && rc == TX_OK; ++it) {
+        for (size_t i = 0; i < count && rc == TX_OK; ++i) {
+            const tx_texture& t = textures[i];
+            rc = tx_compress_device(t.format, t.d_rgba, t.width, t.height, &t.params, t.d_output,
+                                    tx_compressed_size(t.format, t.width, t.height), ctx->stream);
+        }
+    }
+    if (rc == TX_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = TX_ERR_HIP;
+    if (rc == TX_OK && hipEventSynchronize(e1) != hipSuccess) rc = TX_ERR_HIP;
+    if (rc == TX_OK && hipEventElapsedTime(ms_total, e0, e1) != hipSuccess) rc = TX_ERR_HIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+}  // extern "C"
