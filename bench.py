@@ -84,6 +84,294 @@ def time_device(tx, fmt, d_in, w, h, params, d_out, out_len, steps, warmup):
     return ms / steps, wall
 
 
+def _upload_set(tx, items):
+    """items: (format, params, image) -> device buffers and batch descriptors."""
+    bufs, descs = [], []
+    for fmt, p, img in items:
+        h, w = img.shape[:2]
+        d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
+        n = tx.Format(fmt).compressed_size(w, h)
+        d_out = tx.DeviceBuffer(max(n, 16))
+        bufs.append((d_in, d_out, n))
+        descs.append((fmt, p, d_in.ptr, d_out.ptr, w, h))
+    return bufs, descs
+
+
+def _download(bufs):
+    return [d_out.download(n) for _, d_out, n in bufs]
+
+
+def suite_cfg0(tx, texgen, O, threads, steps):
+    """BC1 ClusterFit on the seeded 512x512 texture."""
+    W = H = 512
+    img = texgen.rgb_texture(W, H, texgen.SEED_BASE)
+    p = tx.Params()
+    d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
+    n = tx.Format.Bc1.compressed_size(W, H)
+    d_out = tx.DeviceBuffer(n)
+    ms, _ = time_device(tx, tx.Format.Bc1, d_in, W, H, p, d_out, n, steps, 2)
+    got = d_out.download()
+    op = O.params(1, p.weights, False)
+    t0 = time.perf_counter()
+    want, stats = O.compress(0, img, W, H, op, threads=threads, stats=True)
+    t_cpu = time.perf_counter() - t0
+    bad = int(np.count_nonzero(np.any(got.reshape(-1, 8) != want.reshape(-1, 8), axis=1)))
+    flops = algorithmic_flops(stats)
+    return {"config": "cfg0 BC1 ClusterFit 512x512", "gpu_mpix_s": round(W * H / 1e6 / (ms / 1e3), 1),
+            "ms": round(ms, 4), "cpu_mpix_s": round(W * H / 1e6 / t_cpu, 3), "cpu_threads": threads,
+            "mismatched_blocks": bad, "blocks": W * H // 16,
+            "valu_tflops": round(flops / (ms / 1e3) / 1e12, 2)}
+
+
+def suite_cfg2(tx, texgen, O, threads, steps):
+    """full mip chain of a seeded 4096x4096 texture, per format."""
+    seed = texgen.SEED_BASE + 2
+    base = texgen.with_alpha_field(texgen.rgb_texture(4096, 4096, seed), seed)
+    chain = texgen.mip_chain(base)
+    px = sum(l.shape[0] * l.shape[1] for l in chain)
+    rows = []
+    cases = [("BC1", 0, tx.Algorithm.RangeFit), ("BC1", 0, tx.Algorithm.ClusterFit),
+             ("BC2", 1, tx.Algorithm.ClusterFit), ("BC3", 2, tx.Algorithm.IterativeClusterFit),
+             ("BC4", 3, tx.Algorithm.ClusterFit)]
+    for name, fmt, alg in cases:
+        p = tx.Params(alg)
+        items = [(fmt, p, l) for l in chain]
+        bufs, descs = _upload_set(tx, items)
+        batch = tx.Batch(descs)
+        batch.run_timed(1)
+        ms_batch = batch.run_timed(steps) / steps
+        got = _download(bufs)
+        ms_plain = tx.time_per_texture(descs, steps) / steps
+        op = O.params(int(alg), p.weights, False)
+        t0 = time.perf_counter()
+        want, flops = [], 0.0
+        for l in chain:
+            w_, st = O.compress(fmt, l, l.shape[1], l.shape[0], op, threads=threads, stats=True)
+            want.append(w_)
+            flops += algorithmic_flops(st)
+        t_cpu = time.perf_counter() - t0
+        bad = sum(int(np.count_nonzero(g != w)) for g, w in zip(got, want))
+        bs = tx.Format(fmt).block_size()
+        traffic = 4 * px + sum(len(w) for w in want)
+        rows.append({"config": "cfg2 %s %s 4096 chain (%d levels)" % (name, alg.name, len(chain)),
+                     "mpix": round(px / 1e6, 3), "batch_ms": round(ms_batch, 4),
+                     "batch_mpix_s": round(px / 1e6 / (ms_batch / 1e3), 1), "per_level_ms": round(ms_plain, 4),
+                     "per_level_mpix_s": round(px / 1e6 / (ms_plain / 1e3), 1),
+                     "cpu_mpix_s": round(px / 1e6 / t_cpu, 3), "cpu_threads": threads,
+                     "gpu_over_cpu": round((px / ms_batch / 1e3) / (px / t_cpu / 1e6), 1),
+                     "mismatched_bytes": bad, "block_bytes": bs,
+                     "valu_tflops": round(flops / (ms_batch / 1e3) / 1e12, 3),
+                     "valu_frac_of_157": round(flops / (ms_batch / 1e3) / 1e12 / VALU_PEAK_TFLOPS, 4),
+                     "hbm_gbs": round(traffic / (ms_batch / 1e3) / 1e9, 1),
+                     "hbm_frac": round(traffic / (ms_batch / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+        batch.close()
+        print("suite: " + json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    return rows
+
+
+def cfg3_items(tx, texgen, count=1000):
+    """`count` textures, sides log-uniform in [3, 2048], full mip
+    chains, formats uniform over BC1/BC3/BC4/BC5, Params::default()."""
+    rng = np.random.Generator(np.random.PCG64(texgen.SEED_BASE + 3))
+    ws = texgen.log_uniform_sides(rng, count, 3, 2048)
+    hs = texgen.log_uniform_sides(rng, count, 3, 2048)
+    fmts = rng.choice([0, 2, 3, 4], size=count)
+    items = []
+    for i in range(count):
+        img = texgen.with_alpha_field(texgen.rgb_texture(int(ws[i]), int(hs[i]), texgen.SEED_BASE + 1000 + i),
+                                      texgen.SEED_BASE + 1000 + i)
+        for lv in texgen.mip_chain(img):
+            items.append((int(fmts[i]), tx.Params(), lv))
+    return items
+
+
+def suite_cfg3(tx, texgen, O, threads, steps, tuning_path, count=1000):
+    items = cfg3_items(tx, texgen, count)
+    px = sum(im.shape[0] * im.shape[1] for _, _, im in items)
+    bufs, descs = _upload_set(tx, items)
+    sig = "cfg3-%d" % count
+    best, tune_ms = tx.tuned(sig, descs, table_path=tuning_path, retune=True)
+    batch = tx.Batch(descs, best)
+    batch.run_timed(1)
+    ms_batch = batch.run_timed(steps) / steps
+    got = _download(bufs)
+    default = tx.Batch(descs)
+    default.run_timed(1)
+    ms_default = default.run_timed(steps) / steps
+    ms_plain = tx.time_per_texture(descs, steps) / steps
+    plain = _download(bufs)
+    mismatch_vs_plain = sum(int(np.count_nonzero(a != b)) for a, b in zip(got, plain))
+    # oracle check on a seeded sample of textures
+    rng = np.random.default_rng(7)
+    sample = rng.choice(len(items), size=min(200, len(items)), replace=False)
+    bad = 0
+    for i in sample:
+        fmt, p, im = items[i]
+        want = O.compress(fmt, im, im.shape[1], im.shape[0], O.params(int(p.algorithm), p.weights, False),
+                          threads=threads)
+        bad += int(np.count_nonzero(got[i] != want))
+    return {"config": "cfg3 %d textures + mip chains (%d levels), BC1/BC3/BC4/BC5 default params" % (count, len(items)),
+            "mpix": round(px / 1e6, 2), "batch_ms": round(ms_batch, 3),
+            "batch_mpix_s": round(px / 1e6 / (ms_batch / 1e3), 1), "batch_default_tuning_ms": round(ms_default, 3),
+            "per_texture_ms": round(ms_plain, 3), "per_texture_mpix_s": round(px / 1e6 / (ms_plain / 1e3), 1),
+            "batch_speedup_vs_per_texture": round(ms_plain / ms_batch, 2), "tuning": best.as_dict(),
+            "autotune_best_ms": round(tune_ms, 3), "mismatched_bytes_vs_per_texture": mismatch_vs_plain,
+            "oracle_sample_textures": int(len(sample)), "oracle_sample_mismatched_bytes": bad}
+
+
+def cfg4_pool(tx, texgen, count=24):
+    """configs[4] job inputs: seeded textures, side log-uniform in [64, 4096],
+    format uniform over BC1/BC3/BC5, IterativeClusterFit."""
+    rng = np.random.Generator(np.random.PCG64(texgen.SEED_BASE + 4))
+    sides = texgen.log_uniform_sides(rng, count, 64, 4096)
+    fmts = rng.choice([0, 2, 4], size=count)
+    pool = []
+    for i in range(count):
+        s = int(sides[i])
+        img = texgen.with_alpha_field(texgen.rgb_texture(s, s, texgen.SEED_BASE + 4000 + i), texgen.SEED_BASE + 4000 + i)
+        img = texgen.normal_map(s, s, texgen.SEED_BASE + 4000 + i) if fmts[i] == 4 else img
+        pool.append((int(fmts[i]), np.ascontiguousarray(img.reshape(-1)), s))
+    return pool
+
+
+def _pcts(lat_ms):
+    a = np.sort(np.asarray(lat_ms))
+    return {"p50_ms": round(float(np.percentile(a, 50)), 3), "p99_ms": round(float(np.percentile(a, 99)), 3),
+            "mean_ms": round(float(a.mean()), 3)}
+
+
+def serve_run(tx, pool, arrivals, picks, params, server_kw):
+    """Jobs arrive at `arrivals` (s, relative); returns per-job latencies and
+    the wall time from first arrival to last completion."""
+    import threading
+    srv = tx.Server(**server_kw)
+    n = len(arrivals)
+    ids = [None] * n
+    t_arr = [0] * n
+    ready = [threading.Event() for _ in range(n)]
+    done_dev, done_host, launch_jobs = [0] * n, [0] * n, [0] * n
+    t0 = time.monotonic_ns() + 20_000_000
+
+    def submitter():
+        for k in range(n):
+            target = t0 + int(arrivals[k] * 1e9)
+            while True:
+                d = target - time.monotonic_ns()
+                if d <= 0:
+                    break
+                time.sleep(min(d / 1e9, 0.001) if d > 200_000 else 0)
+            fmt, img, s = pool[picks[k]]
+            t_arr[k] = time.monotonic_ns()
+            while True:
+                try:
+                    ids[k] = srv.submit(fmt, img, s, s, params)
+                    break
+                except tx.TexpressoError as e:
+                    if e.code != -9:
+                        raise
+                    time.sleep(0.0002)
+            ready[k].set()
+
+    def collector():
+        for k in range(n):
+            ready[k].wait()
+            info = srv.wait(ids[k])
+            fmt, img, s = pool[picks[k]]
+            srv.fetch(ids[k], tx.Format(fmt).compressed_size(s, s))
+            done_host[k] = time.monotonic_ns()
+            done_dev[k] = info.t_done_ns
+            launch_jobs[k] = info.launch_jobs
+            srv.release(ids[k])
+
+    ts = threading.Thread(target=submitter)
+    tc = threading.Thread(target=collector)
+    ts.start()
+    tc.start()
+    ts.join()
+    tc.join()
+    srv.close()
+    lat_dev = [(done_dev[k] - t_arr[k]) / 1e6 for k in range(n)]
+    lat_host = [(done_host[k] - t_arr[k]) / 1e6 for k in range(n)]
+    wall = (max(done_host) - min(t_arr)) / 1e9
+    return lat_dev, lat_host, wall, float(np.mean(launch_jobs))
+
+
+def sequential_run(tx, pool, arrivals, picks, params):
+    """The plain path: one tx_compress (upload, encode, download) per job,
+    in arrival order, one at a time."""
+    n = len(arrivals)
+    outs = {}
+    lat = []
+    t0 = time.monotonic_ns() + 20_000_000
+    first = None
+    last = 0
+    for k in range(n):
+        target = t0 + int(arrivals[k] * 1e9)
+        now = time.monotonic_ns()
+        if now < target:
+            time.sleep((target - now) / 1e9)
+        fmt, img, s = pool[picks[k]]
+        f = tx.Format(fmt)
+        if fmt not in outs or outs[fmt][1] != s:
+            outs[fmt] = (np.zeros(f.compressed_size(s, s), np.uint8), s)
+        f.compress(img, s, s, params, outs[fmt][0])
+        last = time.monotonic_ns()
+        first = target if first is None else first
+        lat.append((last - target) / 1e6)
+    return lat, (last - first) / 1e9
+
+
+def suite_cfg4(tx, texgen, rates, pool_size=24, seconds=2.0):
+    pool = cfg4_pool(tx, texgen, pool_size)
+    params = tx.Params(tx.Algorithm.IterativeClusterFit)
+    rows = []
+    for rate in rates:
+        n = int(max(300, min(2000, rate * seconds)))
+        rng = np.random.default_rng(int(rate))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+        picks = rng.integers(0, len(pool), size=n)
+        mpix = float(sum(pool[p][2] ** 2 for p in picks)) / 1e6
+        lat_dev, lat_host, wall, avg_batch = serve_run(
+            tx, pool, arrivals, picks, params,
+            dict(hbm_budget=16 << 30, max_inflight=3, max_jobs_per_launch=64))
+        lat_seq, wall_seq = sequential_run(tx, pool, arrivals, picks, params)
+        row = {"config": "cfg4 serving, Poisson %g jobs/s, %d jobs, BC1/BC3/BC5 IterativeClusterFit, sides 64-4096" %
+                         (rate, n),
+               "offered_mpix_s": round(mpix / arrivals[-1], 1),
+               "server_mpix_s": round(mpix / wall, 1), "server_device_latency": _pcts(lat_dev),
+               "server_latency_incl_fetch": _pcts(lat_host), "avg_jobs_per_launch": round(avg_batch, 2),
+               "sequential_mpix_s": round(mpix / wall_seq, 1), "sequential_latency": _pcts(lat_seq)}
+        rows.append(row)
+        print("suite: " + json.dumps(row), file=sys.stderr, flush=True)
+    return rows
+
+
+def run_suite(args):
+    import texpresso_amd as tx
+    from oracle import oracle as O
+    from texpresso_amd import texgen
+    threads = cpu_threads()
+    which = args.suite.split(",")
+    out = []
+    if "cfg0" in which:
+        out.append(suite_cfg0(tx, texgen, O, threads, args.steps))
+        print(json.dumps(out[-1]), flush=True)
+    if "cfg2" in which:
+        for r in suite_cfg2(tx, texgen, O, threads, args.steps):
+            out.append(r)
+            print(json.dumps(r), flush=True)
+    if "cfg3" in which:
+        out.append(suite_cfg3(tx, texgen, O, threads, args.steps, args.tuning_table, args.cfg3_count))
+        print(json.dumps(out[-1]), flush=True)
+    if "cfg4" in which:
+        for r in suite_cfg4(tx, texgen, [float(x) for x in args.qps.split(",")]):
+            out.append(r)
+            print(json.dumps(r), flush=True)
+    if args.suite_out:
+        with open(args.suite_out, "w") as f:
+            json.dump(out, f, indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,7 +382,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
+    ap.add_argument("--suite", default="", help="comma list of cfg0,cfg2,cfg3: per-config lines instead of the "
+                                                 "default line")
+    ap.add_argument("--suite-out", default="", help="write the suite results to this JSON file")
+    ap.add_argument("--tuning-table", default=os.path.join(ROOT, "texpresso_amd", "tuning.json"))
+    ap.add_argument("--cfg3-count", type=int, default=1000)
+    ap.add_argument("--qps", default="100,250,500,1000,2000", help="cfg4 arrival-rate sweep (jobs/s)")
     args = ap.parse_args()
+    if args.suite:
+        if args.gpus != 1:
+            sys.exit(2)
+        run_suite(args)
+        return
     if args.copy_calibration:
         import texpresso_amd as tx
         n = args.copy_calibration << 20

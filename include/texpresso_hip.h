@@ -152,6 +152,62 @@ int tx_batch_autotune(const tx_texture* textures, size_t count, const tx_batch_t
  * stream, `iters` passes, elapsed ms (HIP events). */
 int tx_util_time_per_texture(const tx_texture* textures, size_t count, int iters, float* ms_total);
 
+/* ---------------------------------------------------------------------
+ * Serving (no reference counterpart).  A scheduler thread admits submitted
+ * jobs in arrival order while their source + output fit in the device memory
+ * budget, uploads the sources into a device arena, and folds every admitted
+ * job into the next persistent-kernel launch on a ring of streams, so jobs
+ * arriving while others run are batched into launches that overlap on the
+ * device.  Outputs stay device-resident until fetched and released.  Each
+ * job's bytes are identical to a plain tx_compress of the same texture.
+ */
+typedef struct {
+    size_t hbm_budget;             /* device bytes for job sources + outputs */
+    uint32_t max_inflight;         /* concurrent launches (streams), 1..8 */
+    uint32_t max_jobs_per_launch;  /* jobs folded into one launch */
+    uint32_t max_pending;          /* submissions waiting for admission */
+    tx_batch_tuning tuning;        /* chunking and grid of each launch */
+} tx_server_config;
+
+typedef struct {
+    int32_t format; /* tx_format */
+    tx_params params;
+    const uint8_t* rgba; /* host RGBA8, width*height*4; valid until the job is done */
+    uint32_t width, height;
+} tx_job;
+
+/* Job states */
+#define TX_JOB_PENDING 0
+#define TX_JOB_ADMITTED 1
+#define TX_JOB_RUNNING 2
+#define TX_JOB_DONE 3
+
+typedef struct {
+    int32_t status;                                       /* TX_JOB_* or a negative error */
+    uint64_t t_submit_ns, t_admit_ns, t_launch_ns, t_done_ns; /* CLOCK_MONOTONIC */
+    const uint8_t* d_output;                              /* device-resident result */
+    size_t out_len;
+    uint32_t launch_id;   /* the launch that ran the job */
+    uint32_t launch_jobs; /* jobs batched into that launch */
+} tx_job_info;
+
+typedef struct tx_server tx_server;
+
+tx_server_config tx_server_config_default(void);
+int tx_server_create(tx_server** server, const tx_server_config* config);
+/* TX_ERR_BUSY when max_pending submissions wait; TX_ERR_NOMEM when the job
+ * alone exceeds the budget; TX_ERR_FORMAT/INVALID for a malformed job. */
+int tx_server_submit(tx_server* server, const tx_job* job, uint64_t* job_id);
+/* Waits up to timeout_ms (< 0: forever) for the job; TX_ERR_BUSY on timeout. */
+int tx_server_wait(tx_server* server, uint64_t job_id, int timeout_ms, tx_job_info* info);
+int tx_server_query(tx_server* server, uint64_t job_id, tx_job_info* info);
+/* Copies a finished job's output to host memory. */
+int tx_server_fetch(tx_server* server, uint64_t job_id, uint8_t* output, size_t out_len);
+/* Frees the job's device memory (after completion). */
+int tx_server_release(tx_server* server, uint64_t job_id);
+/* Stops admitting, waits for running launches, frees everything. */
+int tx_server_destroy(tx_server* server);
+
 /* Host copy of the four single-colour lookup tables compiled into the device
  * code (lib/src/colourfit/single_lut.rs), 6 bytes per entry
  * (start0,end0,err0,start1,end1,err1), order 5_3, 6_3, 5_4, 6_4. */

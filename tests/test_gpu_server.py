@@ -1,0 +1,118 @@
+"""Continuous-batching server: every job's bytes are the same when it runs
+alone and when it is batched with others (and equal the oracle's); admission
+by device-memory budget; a malformed job fails alone; jobs submitted while
+others run are batched into shared launches."""
+import time
+
+import numpy as np
+import pytest
+
+import texpresso_amd as tx
+from texpresso_amd import texgen
+
+pytestmark = pytest.mark.gpu
+
+
+def make_jobs(seed, n, lo=8, hi=200):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        w = int(rng.integers(lo, hi + 1))
+        h = int(rng.integers(lo, hi + 1))
+        fmt = int(rng.choice([0, 2, 4, 1, 3]))
+        alg = tx.Algorithm(int(rng.choice([2, 2, 1, 0])))
+        img = texgen.with_alpha_field(texgen.rgb_texture(w, h, seed * 1000 + i), seed + i)
+        jobs.append((fmt, tx.Params(alg), img, w, h))
+    return jobs
+
+
+def run_alone(srv, jobs):
+    outs = []
+    for fmt, p, img, w, h in jobs:
+        jid = srv.submit(fmt, img, w, h, p)
+        info = srv.wait(jid)
+        assert info.status == 3
+        outs.append(srv.fetch(jid, tx.Format(fmt).compressed_size(w, h)))
+        srv.release(jid)
+    return outs
+
+
+def run_mixed(srv, jobs):
+    ids = [srv.submit(fmt, img, w, h, p) for fmt, p, img, w, h in jobs]
+    outs, infos = [], []
+    for jid, (fmt, p, img, w, h) in zip(ids, jobs):
+        infos.append(srv.wait(jid))
+        outs.append(srv.fetch(jid, tx.Format(fmt).compressed_size(w, h)))
+        srv.release(jid)
+    return outs, infos
+
+
+def test_result_same_alone_and_in_the_mix(gpu, oracle):
+    jobs = make_jobs(1, 40)
+    srv = tx.Server(hbm_budget=256 << 20, max_inflight=3, max_jobs_per_launch=16)
+    alone = run_alone(srv, jobs)
+    mixed, infos = run_mixed(srv, jobs)
+    srv.close()
+    for k, (a, m, (fmt, p, img, w, h)) in enumerate(zip(alone, mixed, jobs)):
+        assert np.array_equal(a, m), "job %d differs alone vs mixed" % k
+        want = oracle.compress(fmt, img, w, h, oracle.params(int(p.algorithm), p.weights, False))
+        assert np.array_equal(m, want), "job %d differs from the oracle" % k
+    # jobs submitted together were folded into shared launches
+    assert max(i.launch_jobs for i in infos) > 1
+    for i in infos:
+        assert i.t_submit_ns <= i.t_admit_ns <= i.t_launch_ns <= i.t_done_ns
+
+
+def test_admission_waits_for_device_memory(gpu, oracle):
+    # a 1 MiB budget holds only a few jobs at a time: the rest wait for releases
+    jobs = make_jobs(2, 24, lo=60, hi=120)
+    srv = tx.Server(hbm_budget=1 << 20, max_inflight=2, max_jobs_per_launch=8)
+    ids = [srv.submit(fmt, img, w, h, p) for fmt, p, img, w, h in jobs]
+    outs = []
+    for jid, (fmt, p, img, w, h) in zip(ids, jobs):
+        info = srv.wait(jid, timeout_ms=60000)
+        assert info.status == 3
+        outs.append(srv.fetch(jid, tx.Format(fmt).compressed_size(w, h)))
+        srv.release(jid)  # frees memory so later jobs can be admitted
+    srv.close()
+    for o, (fmt, p, img, w, h) in zip(outs, jobs):
+        assert np.array_equal(o, oracle.compress(fmt, img, w, h, oracle.params(int(p.algorithm), p.weights, False)))
+
+
+def test_bad_jobs_fail_alone(gpu):
+    srv = tx.Server(hbm_budget=4 << 20)
+    img = texgen.rgb_texture(64, 64, 3)
+    ok = srv.submit(0, img, 64, 64, tx.Params())
+    with pytest.raises(tx.TexpressoError) as e:
+        srv.submit(9, img, 64, 64, tx.Params())
+    assert e.value.code == -1
+    with pytest.raises(tx.TexpressoError) as e:
+        srv.submit(0, img, 0, 64, tx.Params())
+    assert e.value.code == -10
+    big = np.zeros((2048, 2048, 4), np.uint8)
+    with pytest.raises(tx.TexpressoError) as e:
+        srv.submit(0, big, 2048, 2048, tx.Params())  # larger than the whole budget
+    assert e.value.code == -11
+    assert srv.wait(ok).status == 3
+    srv.release(ok)
+    srv.close()
+
+
+def test_wait_timeout_and_pending_bound(gpu):
+    srv = tx.Server(hbm_budget=64 << 20, max_pending=2, max_inflight=1, max_jobs_per_launch=1)
+    # each job runs ~2 ms, far longer than a submit, so the pending bound is hit
+    img = texgen.with_alpha_field(texgen.rgb_texture(2048, 2048, 4), 4)
+    p = tx.Params(tx.Algorithm.IterativeClusterFit)
+    ids = []
+    busy = False
+    for _ in range(16):
+        try:
+            ids.append(srv.submit(2, img, 2048, 2048, p))
+        except tx.TexpressoError as e:
+            assert e.code == -9
+            busy = True
+    assert busy
+    for jid in ids:
+        assert srv.wait(jid).status == 3
+        srv.release(jid)
+    srv.close()

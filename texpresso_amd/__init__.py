@@ -70,6 +70,40 @@ class _TxTuning(ctypes.Structure):
     ]
 
 
+class _TxServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_size_t),
+        ("max_inflight", ctypes.c_uint32),
+        ("max_jobs_per_launch", ctypes.c_uint32),
+        ("max_pending", ctypes.c_uint32),
+        ("tuning", _TxTuning),
+    ]
+
+
+class _TxJob(ctypes.Structure):
+    _fields_ = [
+        ("format", ctypes.c_int32),
+        ("params", _TxParams),
+        ("rgba", ctypes.c_void_p),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+    ]
+
+
+class _TxJobInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("t_submit_ns", ctypes.c_uint64),
+        ("t_admit_ns", ctypes.c_uint64),
+        ("t_launch_ns", ctypes.c_uint64),
+        ("t_done_ns", ctypes.c_uint64),
+        ("d_output", ctypes.c_void_p),
+        ("out_len", ctypes.c_size_t),
+        ("launch_id", ctypes.c_uint32),
+        ("launch_jobs", ctypes.c_uint32),
+    ]
+
+
 class Params:
     """lib/src/lib.rs:76-100; defaults ClusterFit, perceptual weights, no alpha weighting."""
 
@@ -143,6 +177,14 @@ def lib():
                                           ctypes.POINTER(_TxTuning), ctypes.POINTER(ctypes.c_float)]),
             "tx_util_time_per_texture": (c_int, [ctypes.POINTER(_TxTexture), c_sz, c_int,
                                                  ctypes.POINTER(ctypes.c_float)]),
+            "tx_server_config_default": (_TxServerConfig, []),
+            "tx_server_create": (c_int, [ctypes.POINTER(c_vp), ctypes.POINTER(_TxServerConfig)]),
+            "tx_server_submit": (c_int, [c_vp, ctypes.POINTER(_TxJob), ctypes.POINTER(ctypes.c_uint64)]),
+            "tx_server_wait": (c_int, [c_vp, ctypes.c_uint64, c_int, ctypes.POINTER(_TxJobInfo)]),
+            "tx_server_query": (c_int, [c_vp, ctypes.c_uint64, ctypes.POINTER(_TxJobInfo)]),
+            "tx_server_fetch": (c_int, [c_vp, ctypes.c_uint64, c_vp, c_sz]),
+            "tx_server_release": (c_int, [c_vp, ctypes.c_uint64]),
+            "tx_server_destroy": (c_int, [c_vp]),
             "tx_error_string": (ctypes.c_char_p, [c_int]),
             "tx_device_arch": (ctypes.c_char_p, []),
         }
@@ -404,6 +446,88 @@ def autotune(textures, candidates=None, iters=3):
         cand, nc = None, 0
     _check(lib().tx_batch_autotune(arr, n, cand, nc, iters, ctypes.byref(best), ctypes.byref(ms)), "autotune")
     return Tuning._from_c(best), ms.value
+
+
+class JobInfo:
+    """Result record of a served job (timestamps in CLOCK_MONOTONIC ns, the
+    clock of time.monotonic_ns())."""
+
+    def __init__(self, c):
+        self.status = c.status
+        self.t_submit_ns, self.t_admit_ns = c.t_submit_ns, c.t_admit_ns
+        self.t_launch_ns, self.t_done_ns = c.t_launch_ns, c.t_done_ns
+        self.d_output = c.d_output
+        self.out_len = c.out_len
+        self.launch_id, self.launch_jobs = c.launch_id, c.launch_jobs
+
+
+class Server:
+    """Continuous-batching compression server (tx_server_*): submit host
+    textures, jobs are admitted by free device memory and folded into
+    persistent-kernel launches that overlap on the GPU."""
+
+    def __init__(self, hbm_budget=None, max_inflight=None, max_jobs_per_launch=None, max_pending=None,
+                 tuning=None):
+        cfg = lib().tx_server_config_default()
+        if hbm_budget is not None:
+            cfg.hbm_budget = int(hbm_budget)
+        if max_inflight is not None:
+            cfg.max_inflight = int(max_inflight)
+        if max_jobs_per_launch is not None:
+            cfg.max_jobs_per_launch = int(max_jobs_per_launch)
+        if max_pending is not None:
+            cfg.max_pending = int(max_pending)
+        if tuning is not None:
+            t = tuning._c()
+            t.use_graph = 0
+            cfg.tuning = t
+        h = ctypes.c_void_p()
+        _check(lib().tx_server_create(ctypes.byref(h), ctypes.byref(cfg)), "server create")
+        self._h = h.value
+        self._keep = {}
+
+    def submit(self, fmt, rgba, width, height, params):
+        """Queue one texture; `rgba` (numpy uint8, width*height*4) is kept alive
+        until the job is released.  Returns the job id."""
+        import numpy as np
+        arr = np.ascontiguousarray(rgba, dtype=np.uint8).reshape(-1)
+        job = _TxJob(int(fmt), params._c(), arr.ctypes.data, width, height)
+        jid = ctypes.c_uint64()
+        _check(lib().tx_server_submit(self._h, ctypes.byref(job), ctypes.byref(jid)), "submit")
+        self._keep[jid.value] = arr
+        return jid.value
+
+    def wait(self, jid, timeout_ms=-1):
+        info = _TxJobInfo()
+        _check(lib().tx_server_wait(self._h, jid, timeout_ms, ctypes.byref(info)), "wait")
+        return JobInfo(info)
+
+    def query(self, jid):
+        info = _TxJobInfo()
+        _check(lib().tx_server_query(self._h, jid, ctypes.byref(info)), "query")
+        return JobInfo(info)
+
+    def fetch(self, jid, out_len):
+        import numpy as np
+        out = np.zeros(out_len, np.uint8)
+        _check(lib().tx_server_fetch(self._h, jid, out.ctypes.data, out_len), "fetch")
+        return out
+
+    def release(self, jid):
+        _check(lib().tx_server_release(self._h, jid), "release")
+        self._keep.pop(jid, None)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tx_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 TUNING_TABLE = os.path.join(HERE, "tuning.json")

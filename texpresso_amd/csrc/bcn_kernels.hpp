@@ -28,6 +28,22 @@ struct BlockListArgs {
     uint32_t weigh_alpha;
 };
 
+// One texture of a persistent batch launch, as the device reads it.
+struct BatchTex {
+    const uint8_t* rgba;
+    uint8_t* out;
+    uint32_t width, height, blocks_wide, nblocks;
+    uint32_t variant;  // 3*format + algorithm (BC4: 9, BC5: 10)
+    uint32_t pad;
+    float wr, wg, wb;
+    uint32_t weigh_alpha;
+};
+
+// Counter reset + persistent kernel over `nchunks` chunks {texture, first
+// block, block count, 0}; `waves_per_group` is 1, 2 or 4.
+hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nchunks, uint32_t* d_counter,
+                            uint32_t grid, uint32_t waves_per_group, hipStream_t s);
+
 hipError_t launch_encode(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s);
 hipError_t launch_decode(int format, const uint8_t* data, uint32_t w, uint32_t h, uint8_t* out, hipStream_t s);

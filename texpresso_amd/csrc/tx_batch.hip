@@ -16,20 +16,13 @@
 
 namespace txd {
 
-// One texture of a batch, as the device sees it.
-struct BatchTex {
-    const uint8_t* rgba;
-    uint8_t* out;
-    uint32_t width, height, blocks_wide, nblocks;
-    uint32_t variant;  // 3*format + algorithm (BC4/BC5: algorithm 0)
-    uint32_t pad;
-    FitParams fp;
-};
+using txk::BatchTex;
 
 template <int FMT, int ALG>
 __device__ __forceinline__ void run_chunk(const BatchTex& T, uint32_t first, uint32_t count, GroupLds& L, int g,
                                           int l)
 {
+    const FitParams fp{T.wr, T.wg, T.wb, T.weigh_alpha};
     for (uint32_t off = 0; off < count; off += 4) {
         const uint32_t b = first + off + (uint32_t)g;
         const bool live = off + (uint32_t)g < count;
@@ -38,7 +31,7 @@ __device__ __forceinline__ void run_chunk(const BatchTex& T, uint32_t first, uin
         bool valid;
         const uint32_t px = fetch_pixel(T.rgba, T.width, T.height, bx, by, live, l, valid);
         uint32_t w[4];
-        encode_block<FMT, ALG>(px, valid, T.fp, L, g, l, w);
+        encode_block<FMT, ALG>(px, valid, fp, L, g, l, w);
         if (live && l == 0) {
             if (FMT == BC1 || FMT == BC4)
                 *reinterpret_cast<uint2*>(T.out + (size_t)b * 8u) = make_uint2(w[0], w[1]);
@@ -89,6 +82,28 @@ __global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
 
 }  // namespace txd
 
+namespace txk {
+
+hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nchunks, uint32_t* d_counter,
+                            uint32_t grid, uint32_t waves_per_group, hipStream_t s)
+{
+    hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
+    switch (waves_per_group) {
+    case 4:
+        hipLaunchKernelGGL(txd::k_batch<4>, dim3(grid), dim3(256), 0, s, d_tex, d_chunks, nchunks, d_counter);
+        break;
+    case 2:
+        hipLaunchKernelGGL(txd::k_batch<2>, dim3(grid), dim3(128), 0, s, d_tex, d_chunks, nchunks, d_counter);
+        break;
+    default:
+        hipLaunchKernelGGL(txd::k_batch<1>, dim3(grid), dim3(64), 0, s, d_tex, d_chunks, nchunks, d_counter);
+        break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace txk
+
 struct tx_batch {
     int device = -1;
     txd::BatchTex* d_tex = nullptr;
@@ -125,22 +140,8 @@ uint32_t cost_class(const tx_texture& t)
 
 hipError_t launch_batch(tx_batch* b, hipStream_t s)
 {
-    hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, b->d_counter);
-    switch (b->tuning.waves_per_group) {
-    case 4:
-        hipLaunchKernelGGL(txd::k_batch<4>, dim3(b->grid), dim3(256), 0, s, b->d_tex, b->d_chunks, b->nchunks,
-                           b->d_counter);
-        break;
-    case 2:
-        hipLaunchKernelGGL(txd::k_batch<2>, dim3(b->grid), dim3(128), 0, s, b->d_tex, b->d_chunks, b->nchunks,
-                           b->d_counter);
-        break;
-    default:
-        hipLaunchKernelGGL(txd::k_batch<1>, dim3(b->grid), dim3(64), 0, s, b->d_tex, b->d_chunks, b->nchunks,
-                           b->d_counter);
-        break;
-    }
-    return hipGetLastError();
+    return txk::launch_batch_raw(b->d_tex, b->d_chunks, b->nchunks, b->d_counter, b->grid, b->tuning.waves_per_group,
+                                 s);
 }
 
 void free_batch(tx_batch* b)
@@ -202,8 +203,10 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
         d.variant = (s.format >= TX_BC4) ? 9u + (uint32_t)(s.format - TX_BC4)
                                          : 3u * (uint32_t)s.format + (uint32_t)s.params.algorithm;
         d.pad = 0;
-        d.fp = txd::FitParams{s.params.weights[0], s.params.weights[1], s.params.weights[2],
-                              s.params.weigh_colour_by_alpha ? 1u : 0u};
+        d.wr = s.params.weights[0];
+        d.wg = s.params.weights[1];
+        d.wb = s.params.weights[2];
+        d.weigh_alpha = s.params.weigh_colour_by_alpha ? 1u : 0u;
         for (uint32_t f = 0; f < d.nblocks; f += t.chunk_blocks) {
             chunks.push_back(make_uint4((uint32_t)i, f, std::min(t.chunk_blocks, d.nblocks - f), 0u));
             cls.push_back(cost_class(s));
