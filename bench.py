@@ -250,6 +250,7 @@ def serve_run(tx, pool, arrivals, picks, params, server_kw):
     t_arr = [0] * n
     ready = [threading.Event() for _ in range(n)]
     done_dev, done_host, launch_jobs = [0] * n, [0] * n, [0] * n
+    phases = [None] * n
     t0 = time.monotonic_ns() + 20_000_000
 
     def submitter():
@@ -281,6 +282,8 @@ def serve_run(tx, pool, arrivals, picks, params, server_kw):
             done_host[k] = time.monotonic_ns()
             done_dev[k] = info.t_done_ns
             launch_jobs[k] = info.launch_jobs
+            phases[k] = ((info.t_admit_ns - info.t_submit_ns) / 1e6, (info.t_launch_ns - info.t_admit_ns) / 1e6,
+                         (info.t_done_ns - info.t_launch_ns) / 1e6)
             srv.release(ids[k])
 
     ts = threading.Thread(target=submitter)
@@ -293,7 +296,11 @@ def serve_run(tx, pool, arrivals, picks, params, server_kw):
     lat_dev = [(done_dev[k] - t_arr[k]) / 1e6 for k in range(n)]
     lat_host = [(done_host[k] - t_arr[k]) / 1e6 for k in range(n)]
     wall = (max(done_host) - min(t_arr)) / 1e9
-    return lat_dev, lat_host, wall, float(np.mean(launch_jobs))
+    ph = np.array(phases)
+    breakdown = {"queue_p50_ms": round(float(np.median(ph[:, 0])), 3),
+                 "admit_to_launch_p50_ms": round(float(np.median(ph[:, 1])), 3),
+                 "launch_to_done_p50_ms": round(float(np.median(ph[:, 2])), 3)}
+    return lat_dev, lat_host, wall, float(np.mean(launch_jobs)), breakdown
 
 
 def sequential_run(tx, pool, arrivals, picks, params):
@@ -331,7 +338,7 @@ def suite_cfg4(tx, texgen, rates, pool_size=24, seconds=2.0):
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
         picks = rng.integers(0, len(pool), size=n)
         mpix = float(sum(pool[p][2] ** 2 for p in picks)) / 1e6
-        lat_dev, lat_host, wall, avg_batch = serve_run(
+        lat_dev, lat_host, wall, avg_batch, breakdown = serve_run(
             tx, pool, arrivals, picks, params,
             dict(hbm_budget=16 << 30, max_inflight=3, max_jobs_per_launch=64))
         lat_seq, wall_seq = sequential_run(tx, pool, arrivals, picks, params)
@@ -340,6 +347,7 @@ def suite_cfg4(tx, texgen, rates, pool_size=24, seconds=2.0):
                "offered_mpix_s": round(mpix / arrivals[-1], 1),
                "server_mpix_s": round(mpix / wall, 1), "server_device_latency": _pcts(lat_dev),
                "server_latency_incl_fetch": _pcts(lat_host), "avg_jobs_per_launch": round(avg_batch, 2),
+               "server_phases": breakdown,
                "sequential_mpix_s": round(mpix / wall_seq, 1), "sequential_latency": _pcts(lat_seq)}
         rows.append(row)
         print("suite: " + json.dumps(row), file=sys.stderr, flush=True)

@@ -678,6 +678,15 @@ __device__ __forceinline__ uint32_t fetch_pixel(const uint8_t* __restrict__ rgba
     return valid ? *reinterpret_cast<const uint32_t*>(rgba + 4 * ((size_t)sy * width + sx)) : 0u;
 }
 
+// BC4 / BC5 blocks: alpha codebooks only (lib.rs:200-204), no LDS.
+template <int FMT>
+__device__ __forceinline__ void encode_alpha_only(uint32_t px, bool valid, int l, uint32_t (&out)[4])
+{
+    out[0] = out[1] = out[2] = out[3] = 0;
+    bc3_alpha((int)(px & 255u), valid, l, out[0], out[1]);
+    if (FMT == BC5) bc3_alpha((int)((px >> 8) & 255u), valid, l, out[2], out[3]);
+}
+
 // Encodes one 4x4 block per 16-lane group (lib.rs:188-234): lane l holds
 // pixel l (`px`, RGBA8 little-endian) and its mask bit (`valid`).  Every lane
 // of the wave must call this (LDS barriers inside); groups without a block
@@ -689,12 +698,13 @@ __device__ __forceinline__ void encode_block(uint32_t px, bool valid, const FitP
 {
     const int gshift = 16 * g;
 
+    if (FMT == BC4 || FMT == BC5) {
+        encode_alpha_only<FMT>(px, valid, l, out);
+        return;
+    }
     out[0] = out[1] = out[2] = out[3] = 0;
     if (FMT == BC2) bc2_alpha(px, valid, l, out[0], out[1]);
     if (FMT == BC3) bc3_alpha((int)(px >> 24), valid, l, out[0], out[1]);
-    if (FMT == BC4 || FMT == BC5) bc3_alpha((int)(px & 255u), valid, l, out[0], out[1]);
-    if (FMT == BC5) bc3_alpha((int)((px >> 8) & 255u), valid, l, out[2], out[3]);
-    if (FMT == BC4 || FMT == BC5) return;
 
     // ---- ColourSet::new, colourset.rs:35-112
     const uint32_t rgb = px & 0xFFFFFFu, alpha = px >> 24;

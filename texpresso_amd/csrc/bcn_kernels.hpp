@@ -39,10 +39,15 @@ struct BatchTex {
     uint32_t weigh_alpha;
 };
 
-// Counter reset + persistent kernel over `nchunks` chunks {texture, first
-// block, block count, 0}; `waves_per_group` is 1, 2 or 4.
-hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nchunks, uint32_t* d_counter,
-                            uint32_t grid, uint32_t waves_per_group, hipStream_t s);
+// Counter reset + persistent kernels over chunks {texture, first block,
+// block count, 0}: the first `nheavy` chunks (any format) go to the LDS
+// encoder (`grid` workgroups of `waves_per_group` = 1, 2 or 4 waves), the
+// next `nlight` (BC4/BC5 only) to the alpha-only kernel (`grid_light`
+// 4-wave workgroups), on `s_light` when given (fork/join through the two
+// events) or after the heavy kernel on `s`.  d_counter holds two counters.
+hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nheavy, uint32_t nlight,
+                            uint32_t* d_counter, uint32_t grid, uint32_t grid_light, uint32_t waves_per_group,
+                            hipStream_t s, hipStream_t s_light, hipEvent_t fork, hipEvent_t join);
 
 hipError_t launch_encode(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s);

@@ -76,28 +76,103 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch(const BatchTex* __restrict
     }
 }
 
-// Clears the work counter with a kernel (a memset node is not replay-safe
+__device__ __forceinline__ uint32_t chunk_pixel(const BatchTex& T, uint32_t first, uint32_t count, uint32_t off, int g,
+                                                int l, bool& valid)
+{
+    const uint32_t b = first + off + (uint32_t)g;
+    const bool live = off + (uint32_t)g < count;
+    const uint32_t bx = live ? b % T.blocks_wide : 0u;
+    const uint32_t by = live ? b / T.blocks_wide : 0u;
+    return fetch_pixel(T.rgba, T.width, T.height, bx, by, live, l, valid);
+}
+
+// Memory-bound: the next four blocks' pixels are fetched before the current
+// four are encoded, so each wave keeps two loads in flight.
+template <int FMT>
+__device__ __forceinline__ void run_chunk_light(const BatchTex& T, uint32_t first, uint32_t count, int g, int l)
+{
+    bool v_next;
+    uint32_t px_next = chunk_pixel(T, first, count, 0, g, l, v_next);
+    for (uint32_t off = 0; off < count; off += 4) {
+        const uint32_t b = first + off + (uint32_t)g;
+        const bool live = off + (uint32_t)g < count;
+        const bool valid = v_next;
+        const uint32_t px = px_next;
+        px_next = chunk_pixel(T, first, count, off + 4, g, l, v_next);
+        uint32_t w[4];
+        encode_alpha_only<FMT>(px, valid, l, w);
+        if (live && l == 0) {
+            if (FMT == BC4)
+                *reinterpret_cast<uint2*>(T.out + (size_t)b * 8u) = make_uint2(w[0], w[1]);
+            else
+                *reinterpret_cast<uint4*>(T.out + (size_t)b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+// Persistent kernel for the alpha-only formats (BC4/BC5): no LDS and few
+// registers, so many waves stay resident to hide memory latency.  Takes
+// chunks [first, first + nchunks) through its own counter.
+__global__ __launch_bounds__(256) void k_batch_light(const BatchTex* __restrict__ tex,
+                                                     const uint4* __restrict__ chunks, uint32_t first,
+                                                     uint32_t nchunks, uint32_t* __restrict__ counter)
+{
+    const int lane = (int)threadIdx.x & 63;
+    const int g = lane >> 4, l = lane & 15;
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(counter, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= nchunks) break;
+        const uint4 ch = chunks[first + c];
+        const BatchTex& T = tex[ch.x];
+        if (T.variant == 9u) run_chunk_light<BC4>(T, ch.y, ch.z, g, l);
+        else run_chunk_light<BC5>(T, ch.y, ch.z, g, l);
+    }
+}
+
+// Clears the work counters with a kernel (a memset node is not replay-safe
 // in a graph on this platform).
-__global__ void k_reset_counter(uint32_t* counter) { *counter = 0u; }
+__global__ void k_reset_counter(uint32_t* counter)
+{
+    counter[0] = 0u;
+    counter[1] = 0u;
+}
 
 }  // namespace txd
 
 namespace txk {
 
-hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nchunks, uint32_t* d_counter,
-                            uint32_t grid, uint32_t waves_per_group, hipStream_t s)
+hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nheavy, uint32_t nlight,
+                            uint32_t* d_counter, uint32_t grid, uint32_t grid_light, uint32_t waves_per_group,
+                            hipStream_t s, hipStream_t s_light, hipEvent_t fork, hipEvent_t join)
 {
     hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
-    switch (waves_per_group) {
-    case 4:
-        hipLaunchKernelGGL(txd::k_batch<4>, dim3(grid), dim3(256), 0, s, d_tex, d_chunks, nchunks, d_counter);
-        break;
-    case 2:
-        hipLaunchKernelGGL(txd::k_batch<2>, dim3(grid), dim3(128), 0, s, d_tex, d_chunks, nchunks, d_counter);
-        break;
-    default:
-        hipLaunchKernelGGL(txd::k_batch<1>, dim3(grid), dim3(64), 0, s, d_tex, d_chunks, nchunks, d_counter);
-        break;
+    hipStream_t sl = s;
+    if (nlight && s_light) {
+        if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s_light, fork, 0) != hipSuccess)
+            return hipErrorUnknown;
+        sl = s_light;
+    }
+    if (nlight)
+        hipLaunchKernelGGL(txd::k_batch_light, dim3(grid_light), dim3(256), 0, sl, d_tex, d_chunks, nheavy, nlight,
+                           d_counter + 1);
+    if (nheavy) {
+        switch (waves_per_group) {
+        case 4:
+            hipLaunchKernelGGL(txd::k_batch<4>, dim3(grid), dim3(256), 0, s, d_tex, d_chunks, nheavy, d_counter);
+            break;
+        case 2:
+            hipLaunchKernelGGL(txd::k_batch<2>, dim3(grid), dim3(128), 0, s, d_tex, d_chunks, nheavy, d_counter);
+            break;
+        default:
+            hipLaunchKernelGGL(txd::k_batch<1>, dim3(grid), dim3(64), 0, s, d_tex, d_chunks, nheavy, d_counter);
+            break;
+        }
+    }
+    if (nlight && s_light) {
+        if (hipEventRecord(join, s_light) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess)
+            return hipErrorUnknown;
     }
     return hipGetLastError();
 }
@@ -109,13 +184,15 @@ struct tx_batch {
     txd::BatchTex* d_tex = nullptr;
     uint4* d_chunks = nullptr;
     uint32_t* d_counter = nullptr;
-    uint32_t nchunks = 0;
+    uint32_t nchunks = 0;  // heavy chunks first, then the alpha-only ones
+    uint32_t nheavy = 0;
     uint64_t nblocks = 0;
     tx_batch_tuning tuning{};
-    uint32_t grid = 0;
+    uint32_t grid = 0, grid_light = 0;
+    hipStream_t light_stream = nullptr;  // runs the alpha-only kernel beside the heavy one
+    hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipStream_t graph_stream = nullptr;
 };
 
 namespace {
@@ -140,14 +217,17 @@ uint32_t cost_class(const tx_texture& t)
 
 hipError_t launch_batch(tx_batch* b, hipStream_t s)
 {
-    return txk::launch_batch_raw(b->d_tex, b->d_chunks, b->nchunks, b->d_counter, b->grid, b->tuning.waves_per_group,
-                                 s);
+    return txk::launch_batch_raw(b->d_tex, b->d_chunks, b->nheavy, b->nchunks - b->nheavy, b->d_counter, b->grid,
+                                 b->grid_light, b->tuning.waves_per_group, s, b->light_stream, b->fork, b->join);
 }
 
 void free_batch(tx_batch* b)
 {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->light_stream) (void)hipStreamDestroy(b->light_stream);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     if (b->d_tex) (void)hipFree(b->d_tex);
     if (b->d_chunks) (void)hipFree(b->d_chunks);
     if (b->d_counter) (void)hipFree(b->d_counter);
@@ -226,13 +306,30 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
     b->device = ctx->device;
     b->tuning = t;
     b->nchunks = (uint32_t)sorted.size();
+    b->nheavy = 0;
+    for (size_t i = 0; i < order.size(); ++i) b->nheavy += cls[order[i]] > 0 ? 1u : 0u;
     b->nblocks = nblocks;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
         free_batch(b);
         return TX_ERR_HIP;
     }
-    b->grid = (uint32_t)prop.multiProcessorCount * t.groups_per_cu;
+    // persistent workgroups: the tuned count, but never more than there are chunks
+    b->grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * t.groups_per_cu,
+                                           std::max<uint64_t>(1, (b->nheavy + t.waves_per_group - 1) /
+                                                                     t.waves_per_group));
+    // alpha-only kernel: 4-wave workgroups, up to 8 per CU
+    const uint32_t nlight = b->nchunks - b->nheavy;
+    b->grid_light = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
+                                                 std::max<uint64_t>(1, (nlight + 3) / 4));
+    if (nlight && b->nheavy) {
+        if (hipStreamCreateWithFlags(&b->light_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+            free_batch(b);
+            return TX_ERR_HIP;
+        }
+    }
     if (hipMalloc(&b->d_tex, sizeof(txd::BatchTex) * std::max<size_t>(1, tex.size())) != hipSuccess ||
         hipMalloc(&b->d_chunks, sizeof(uint4) * std::max<size_t>(1, sorted.size())) != hipSuccess ||
         hipMalloc(&b->d_counter, 256) != hipSuccess) {

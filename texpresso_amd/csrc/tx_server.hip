@@ -170,7 +170,17 @@ uint32_t variant_of(const tx_job& s)
 // Called with the lock held; drops it while enqueueing device work.
 bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_lock<std::mutex>& lock)
 {
-    const uint32_t chunk = cfg.tuning.chunk_blocks;
+    // chunks small enough that every resident wave gets work even when the
+    // launch holds a single small job (latency), capped by the tuned size
+    uint64_t total_blocks = 0;
+    for (uint64_t id : ids) {
+        const Job& j = jobs[id];
+        total_blocks += (uint64_t)((j.spec.width + 3) / 4) * ((j.spec.height + 3) / 4);
+    }
+    const uint64_t waves = (uint64_t)grid * cfg.tuning.waves_per_group;
+    uint64_t want = (total_blocks + waves - 1) / std::max<uint64_t>(1, waves);
+    want = std::max<uint64_t>(4, (want + 3) & ~(uint64_t)3);
+    const uint32_t chunk = (uint32_t)std::min<uint64_t>(cfg.tuning.chunk_blocks, want);
     size_t nchunks = 0;
     std::vector<tx_job> specs;
     std::vector<size_t> in_offs, out_offs;
@@ -233,9 +243,12 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
     if (ok && c && hipMemcpyAsync(L.d_chunks, L.h_chunks, sizeof(uint4) * c, hipMemcpyHostToDevice, L.stream) !=
                        hipSuccess)
         ok = false;
+    // no more persistent workgroups than chunks
+    const uint32_t launch_grid =
+        (uint32_t)std::min<uint64_t>(grid, (c + cfg.tuning.waves_per_group - 1) / cfg.tuning.waves_per_group);
     if (ok && c &&
-        txk::launch_batch_raw(L.d_tex, L.d_chunks, (uint32_t)c, L.d_counter, grid, cfg.tuning.waves_per_group,
-                              L.stream) != hipSuccess)
+        txk::launch_batch_raw(L.d_tex, L.d_chunks, (uint32_t)c, 0u, L.d_counter, launch_grid, 0u,
+                              cfg.tuning.waves_per_group, L.stream, nullptr, nullptr, nullptr) != hipSuccess)
         ok = false;
     if (ok && hipEventRecord(L.done, L.stream) != hipSuccess) ok = false;
     lock.lock();
