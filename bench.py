@@ -396,7 +396,13 @@ def main():
     ap.add_argument("--tuning-table", default=os.path.join(ROOT, "texpresso_amd", "tuning.json"))
     ap.add_argument("--cfg3-count", type=int, default=1000)
     ap.add_argument("--qps", default="100,250,500,1000,2000", help="cfg4 arrival-rate sweep (jobs/s)")
+    ap.add_argument("--pipeline-threshold", type=int, default=-1,
+                    help="blocks from which one texture goes through the binned ClusterFit pipeline "
+                         "(-1: library default; 0: always; a huge value: never)")
     args = ap.parse_args()
+    if args.pipeline_threshold >= 0:
+        import texpresso_amd as tx
+        tx.set_pipeline_threshold(args.pipeline_threshold)
     if args.suite:
         if args.gpus != 1:
             sys.exit(2)

@@ -126,6 +126,9 @@ typedef struct {
     uint32_t waves_per_group;  /* waves per workgroup: 1, 2 or 4 (4 blocks per wave) */
     uint32_t groups_per_cu;    /* persistent workgroups per compute unit */
     uint32_t use_graph;        /* 1: capture the launch in a hipGraph and replay it */
+    uint32_t pipeline;         /* 1: ClusterFit blocks through the binned pipeline (classify, sort by
+                                  colour count, independent per-block searches); 0: one persistent
+                                  kernel with the four blocks of a wave in lockstep */
 } tx_batch_tuning;
 
 typedef struct tx_batch tx_batch;
@@ -226,6 +229,13 @@ int tx_util_copy(void* d_dst, const void* d_src, size_t bytes);
  * library stream, waits, and stores the elapsed milliseconds. */
 int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
                           uint8_t* d_output, size_t out_len, int iters, float* ms_total);
+
+/* ClusterFit textures with at least `min_blocks` blocks are encoded by the
+ * binned pipeline (classify, counting sort by colour count, per-block
+ * independent searches); smaller ones by the single direct kernel.  Both
+ * give identical bytes.  0 = always the pipeline, UINT64_MAX = never.
+ * Process-wide; returns the previous value (default 131072). */
+uint64_t tx_set_pipeline_threshold(uint64_t min_blocks);
 
 /* Measures the device's plain FP32 VALU rate (separate multiply and add, no
  * FMA, no packed math) in lane-operations per second. */

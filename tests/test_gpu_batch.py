@@ -66,7 +66,8 @@ def test_batch_parity_and_graph_replays(gpu, oracle):
     items = make_set(3, 24, 90)
     bufs, descs = upload(items)
     want = expected(oracle, items)
-    batch = tx.Batch(descs, tx.Tuning(chunk_blocks=16, waves_per_group=1, groups_per_cu=4, use_graph=True))
+    batch = tx.Batch(descs, tx.Tuning(chunk_blocks=16, waves_per_group=1, groups_per_cu=4, use_graph=True,
+                                      pipeline=True))
     assert batch.info()["blocks"] == sum(tx.num_blocks(d[4]) * tx.num_blocks(d[5]) for d in descs)
     scratch_img = texgen.rgb_texture(33, 17, 5)
     for rep in range(3):
@@ -80,8 +81,10 @@ def test_batch_parity_and_graph_replays(gpu, oracle):
 
 
 @pytest.mark.parametrize("tuning", [
-    tx.Tuning(4, 1, 2, False), tx.Tuning(32, 2, 6, True), tx.Tuning(256, 4, 3, True), tx.Tuning(64, 1, 12, True),
-], ids=["c4w1", "c32w2", "c256w4", "c64w1g12"])
+    tx.Tuning(4, 1, 2, False, False), tx.Tuning(32, 2, 6, True, False), tx.Tuning(256, 4, 3, True, False),
+    tx.Tuning(64, 1, 12, True, False), tx.Tuning(4, 1, 12, False, True), tx.Tuning(64, 1, 12, True, True),
+    tx.Tuning(256, 1, 12, True, True),
+], ids=["c4w1", "c32w2", "c256w4", "c64w1g12", "pipe-c4-nograph", "pipe-c64", "pipe-c256"])
 def test_batch_identical_across_tunings(gpu, oracle, tuning):
     items = make_set(9, 16, 70)
     bufs, descs = upload(items)

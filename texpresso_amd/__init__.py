@@ -67,6 +67,7 @@ class _TxTuning(ctypes.Structure):
         ("waves_per_group", ctypes.c_uint32),
         ("groups_per_cu", ctypes.c_uint32),
         ("use_graph", ctypes.c_uint32),
+        ("pipeline", ctypes.c_uint32),
     ]
 
 
@@ -164,6 +165,7 @@ def lib():
             "tx_util_download": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_copy": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_valu_probe": (c_int, [ctypes.POINTER(ctypes.c_double)]),
+            "tx_set_pipeline_threshold": (ctypes.c_uint64, [ctypes.c_uint64]),
             "tx_util_time_compress": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
                                               ctypes.POINTER(ctypes.c_float)]),
             "tx_batch_tuning_default": (_TxTuning, []),
@@ -338,6 +340,12 @@ class DeviceBuffer:
             pass
 
 
+def set_pipeline_threshold(min_blocks):
+    """ClusterFit textures with at least `min_blocks` blocks use the binned
+    pipeline (0: always, 2**64-1: never).  Returns the previous value."""
+    return lib().tx_set_pipeline_threshold(int(min_blocks))
+
+
 def time_compress(fmt, d_rgba, width, height, params, d_out, out_len, iters):
     """Milliseconds for `iters` back-to-back device launches (HIP events)."""
     ms = ctypes.c_float()
@@ -349,22 +357,25 @@ def time_compress(fmt, d_rgba, width, height, params, d_out, out_len, iters):
 
 class Tuning:
     """Batch kernel tuning: blocks per queue chunk, waves (x4 blocks) per
-    workgroup, persistent workgroups per CU, hipGraph replay."""
+    workgroup, persistent workgroups per CU, hipGraph replay, and whether
+    ClusterFit blocks go through the binned pipeline."""
 
-    __slots__ = ("chunk_blocks", "waves_per_group", "groups_per_cu", "use_graph")
+    __slots__ = ("chunk_blocks", "waves_per_group", "groups_per_cu", "use_graph", "pipeline")
 
-    def __init__(self, chunk_blocks=64, waves_per_group=1, groups_per_cu=12, use_graph=True):
+    def __init__(self, chunk_blocks=64, waves_per_group=1, groups_per_cu=12, use_graph=True, pipeline=False):
         self.chunk_blocks = int(chunk_blocks)
         self.waves_per_group = int(waves_per_group)
         self.groups_per_cu = int(groups_per_cu)
         self.use_graph = bool(use_graph)
+        self.pipeline = bool(pipeline)
 
     def _c(self):
-        return _TxTuning(self.chunk_blocks, self.waves_per_group, self.groups_per_cu, 1 if self.use_graph else 0)
+        return _TxTuning(self.chunk_blocks, self.waves_per_group, self.groups_per_cu, 1 if self.use_graph else 0,
+                         1 if self.pipeline else 0)
 
     @classmethod
     def _from_c(cls, t):
-        return cls(t.chunk_blocks, t.waves_per_group, t.groups_per_cu, bool(t.use_graph))
+        return cls(t.chunk_blocks, t.waves_per_group, t.groups_per_cu, bool(t.use_graph), bool(t.pipeline))
 
     def as_dict(self):
         return {k: getattr(self, k) for k in self.__slots__}

@@ -524,7 +524,133 @@ __device__ __forceinline__ void group_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// cluster.rs:78-136 (ordering) and cluster.rs:152-274 / 276-417 (search).
+// construct_ordering (cluster.rs:78-136) for the ordering slot `it` of every
+// group with `active` set, then the weighted points and the prefix table of
+// that ordering.  Executed by the whole wave.  Returns false when the new
+// ordering repeats one of slots 0..it-1 (the search then stops, and the
+// prefix table is left untouched).
+__device__ __forceinline__ bool ordering_step(GroupLds& L, int l, int n, bool active, int it, float axx, float axy,
+                                              float axz)
+{
+    float key = FLT_MAX;
+    uint32_t idx = 0;
+    if (l < n) {
+        const float4 q = L.pts[l];
+        key = (q.x * axx + q.y * axy) + q.z * axz;
+        idx = (uint32_t)l;
+    }
+    if (!__builtin_isfinite(key)) key = __builtin_huge_valf();  // non-finite keys tie, after all finite
+    if (active) L.keys[l] = key;
+    group_sync();
+    // stable rank: entries with a smaller key, plus equal keys at lower index
+    int rank = 0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const float km = L.keys[m];
+        rank += (km < key) ? 1 : 0;
+        rank += (m < l && km == key) ? 1 : 0;
+    }
+    if (active) reinterpret_cast<uint8_t*>(L.ord[it])[rank] = (uint8_t)idx;
+    group_sync();
+    bool unique = active;
+    if (active) {
+        const uint4 cur = *reinterpret_cast<const uint4*>(L.ord[it]);
+        for (int p = 0; p < it; ++p) {
+            const uint4 o = *reinterpret_cast<const uint4*>(L.ord[p]);
+            if (o.x == cur.x && o.y == cur.y && o.z == cur.z && o.w == cur.w) unique = false;
+        }
+    }
+    // weighted points in order, then S[a][b] = ((0 + pw[a]) + ...) + pw[b-1]
+    if (unique && l < n) {
+        const uint32_t o = (L.ord[it][l >> 2] >> (8 * (l & 3))) & 255u;
+        const float4 q = L.pts[o];
+        L.pw[l] = make_float4(q.x * q.w, q.y * q.w, q.z * q.w, 1.0f * q.w);
+    }
+    group_sync();
+    if (unique) {
+        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        L.S[txt::tri(l, l)] = s;
+        if (l == 0) L.S[txt::tri(16, 16)] = s;
+        for (int t = 0; t < 16; ++t) {
+            const int bb = l + 1 + t;
+            if (bb <= n) {
+                const float4 w = L.pw[bb - 1];
+                s.x = s.x + w.x; s.y = s.y + w.y; s.z = s.z + w.z; s.w = s.w + w.w;
+                L.S[txt::tri(l, bb)] = s;
+            }
+        }
+    }
+    group_sync();
+    return unique;
+}
+
+// The split search over the current prefix table: every group with `run`
+// set evaluates its `len` candidates (16 lanes, list entries two rounds and
+// prefix sums one round ahead) and returns the first strict minimum
+// (error, code) on all of its lanes.  Executed by the whole wave.
+template <int NC>
+__device__ __forceinline__ void search_step(const GroupLds& L, int l, int n, bool run, const FitParams& fp,
+                                            float& be, uint32_t& bc)
+{
+    const uint32_t mylen = run ? (NC == 4 ? c_cand.len4[n] : c_cand.len3[n]) : 0u;
+    const uint32_t* tab = NC == 4 ? c_cand.t4 : c_cand.t3;
+    const float4 xs = L.S[n];  // xsum_wsum = S[0][n]
+    be = __builtin_huge_valf();
+    bc = 0xFFFFFFFFu;
+    const uint32_t t0 = (uint32_t)l;
+    uint32_t c0 = t0 < mylen ? tab[t0] : 0u;
+    uint32_t c1 = t0 + 16 < mylen ? tab[t0 + 16] : 0u;
+    SplitSums s0 = load_split<NC>(L, c0);
+    for (uint32_t base = 0; wave_any(base < mylen); base += 16) {
+        const uint32_t t = base + (uint32_t)l;
+        const uint32_t c2 = t + 32 < mylen ? tab[t + 32] : 0u;
+        const SplitSums s1 = load_split<NC>(L, c1);
+        if (t < mylen) {
+            float a[3], b[3];
+            const float err = eval_split<NC>(s0, xs, fp, a, b);
+            if (err < be || (err == be && c0 < bc)) {
+                be = err;
+                bc = c0;
+            }
+        }
+        c0 = c1;
+        c1 = c2;
+        s0 = s1;
+    }
+    grp_argmin(be, bc);
+}
+
+// Quantised endpoints of split `code` on the current prefix table.
+template <int NC>
+__device__ __forceinline__ void split_endpoints(const GroupLds& L, int n, uint32_t code, const FitParams& fp,
+                                                float (&sa)[3], float (&sb)[3])
+{
+    eval_split<NC>(load_split<NC>(L, code), L.S[n], fp, sa, sb);
+}
+
+// Index of this lane's pixel for the winning split (cluster.rs:252-273 /
+// 391-416 followed by remap_indices): the three overwriting loops leave a
+// point with the value of the range holding its largest position q >= i.
+template <int NC>
+__device__ __forceinline__ uint32_t split_index(const GroupLds& L, int n, int remap, uint32_t code, int it)
+{
+    if (remap < 0) return 3u;
+    const int i = (int)((code >> 16) & 15u);
+    const int j = (int)(code & 255u) - txt::tri(i, 0);
+    const int k = NC == 4 ? (int)((code >> 8) & 255u) - txt::tri(j, 0) : 0;
+    int qmax = -1;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int pq = (int)((L.ord[it][q >> 2] >> (8 * (q & 3))) & 255u);
+        if (q >= i && q < n && pq == remap) qmax = q;
+    }
+    if (qmax < 0) return 0u;
+    if (qmax < j) return 2u;
+    if (NC == 4 && qmax < k) return 3u;
+    return 1u;
+}
+
+// cluster.rs:152-274 / 276-417 with all groups of the wave in lockstep.
 // Executed by the whole wave; `act` marks the groups running ClusterFit.
 // `best_err`, `lo`, `hi` carry ClusterFit::best_error / best_compressed
 // across compress3 -> compress4.
@@ -541,94 +667,20 @@ __device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, in
     float sa[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};  // best_start / best_end
     float axx = px, axy = py, axz = pz;
     bool done = !act;
-    const uint32_t len = act ? (NC == 4 ? c_cand.len4[n] : c_cand.len3[n]) : 0u;
-    const uint32_t* tab = NC == 4 ? c_cand.t4 : c_cand.t3;
 
     for (int it = 0; it < ITERS; ++it) {
         if (!wave_any(!done)) break;
-        // ---- construct_ordering: stable sort of the projections
-        float key = FLT_MAX;
-        uint32_t idx = 0;
-        if (l < n) {
-            const float4 q = L.pts[l];
-            key = (q.x * axx + q.y * axy) + q.z * axz;
-            idx = (uint32_t)l;
-        }
-        if (!__builtin_isfinite(key)) key = __builtin_huge_valf();  // non-finite keys tie, after all finite
-        if (!done) L.keys[l] = key;
-        group_sync();
-        int rank = 0;
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const float km = L.keys[m];
-            rank += (km < key) ? 1 : 0;
-            rank += (m < l && km == key) ? 1 : 0;
-        }
-        if (!done) reinterpret_cast<uint8_t*>(L.ord[it])[rank] = (uint8_t)idx;
-        group_sync();
-        if (!done) {
-            const uint4 cur = *reinterpret_cast<const uint4*>(L.ord[it]);
-            for (int p = 0; p < it; ++p) {
-                const uint4 o = *reinterpret_cast<const uint4*>(L.ord[p]);
-                if (o.x == cur.x && o.y == cur.y && o.z == cur.z && o.w == cur.w) done = true;
-            }
-        }
-        // ---- weighted points and the prefix table
-        if (!done && l < n) {
-            const uint32_t o = (L.ord[it][l >> 2] >> (8 * (l & 3))) & 255u;
-            const float4 q = L.pts[o];
-            L.pw[l] = make_float4(q.x * q.w, q.y * q.w, q.z * q.w, 1.0f * q.w);
-        }
-        group_sync();
-        if (!done) {
-            float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            L.S[txt::tri(l, l)] = s;
-            if (l == 0) L.S[txt::tri(16, 16)] = s;
-            for (int t = 0; t < 16; ++t) {
-                const int bb = l + 1 + t;
-                if (bb <= n) {
-                    const float4 w = L.pw[bb - 1];
-                    s.x = s.x + w.x; s.y = s.y + w.y; s.z = s.z + w.z; s.w = s.w + w.w;
-                    L.S[txt::tri(l, bb)] = s;
-                }
-            }
-        }
-        group_sync();
-        // ---- split search
-        const float4 xs = L.S[n];  // xsum_wsum = S[0][n]
-        const uint32_t mylen = done ? 0u : len;
-        float be = __builtin_huge_valf();
-        uint32_t bc = 0xFFFFFFFFu;
-        // software pipeline: list entries two rounds ahead, prefix sums one
-        // round ahead (an unused slot reads entry 0, which is in bounds)
-        const uint32_t t0 = (uint32_t)l;
-        uint32_t c0 = t0 < mylen ? tab[t0] : 0u;
-        uint32_t c1 = t0 + 16 < mylen ? tab[t0 + 16] : 0u;
-        SplitSums s0 = load_split<NC>(L, c0);
-        for (uint32_t base = 0; wave_any(base < mylen); base += 16) {
-            const uint32_t t = base + (uint32_t)l;
-            const uint32_t c2 = t + 32 < mylen ? tab[t + 32] : 0u;
-            const SplitSums s1 = load_split<NC>(L, c1);
-            if (t < mylen) {
-                float a[3], b[3];
-                const float err = eval_split<NC>(s0, xs, fp, a, b);
-                if (err < be || (err == be && c0 < bc)) {
-                    be = err;
-                    bc = c0;
-                }
-            }
-            c0 = c1;
-            c1 = c2;
-            s0 = s1;
-        }
-        grp_argmin(be, bc);
-        // ---- iteration control, cluster.rs:222-249
+        if (!ordering_step(L, l, n, !done, it, axx, axy, axz)) done = true;
+        float be;
+        uint32_t bc;
+        search_step<NC>(L, l, n, !done, fp, be, bc);
+        // iteration control, cluster.rs:222-249
         if (!done) {
             if (be < lb_err) {
                 lb_err = be;
                 lb_code = bc;
                 lb_it = it;
-                eval_split<NC>(load_split<NC>(L, bc), xs, fp, sa, sb);
+                split_endpoints<NC>(L, n, bc, fp, sa, sb);
             }
             if (lb_it != it) {
                 done = true;
@@ -640,28 +692,10 @@ __device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, in
         }
     }
 
-    // ---- index assembly, cluster.rs:252-273 / 391-416
     if (act && lb_err < thr) {
-        const int i = (int)((lb_code >> 16) & 15u);
-        const int j = (int)(lb_code & 255u) - txt::tri(i, 0);
-        const int k = NC == 4 ? (int)((lb_code >> 8) & 255u) - txt::tri(j, 0) : 0;
-        uint32_t idx = 3;
-        if (cs.remap >= 0) {
-            // the value a point ends with is set by the last range that
-            // writes it, i.e. by its largest position q >= i in the order
-            int qmax = -1;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int pq = (int)((L.ord[lb_it][q >> 2] >> (8 * (q & 3))) & 255u);
-                if (q >= i && q < n && pq == cs.remap) qmax = q;
-            }
-            if (qmax < 0) idx = 0;
-            else if (qmax < j) idx = 2;
-            else if (NC == 4 && qmax < k) idx = 3;
-            else idx = 1;
-        }
         uint32_t nlo, nhi;
-        write_colour(NC == 4, sa[0], sa[1], sa[2], sb[0], sb[1], sb[2], idx, l, nlo, nhi);
+        write_colour(NC == 4, sa[0], sa[1], sa[2], sb[0], sb[1], sb[2], split_index<NC>(L, n, cs.remap, lb_code, lb_it),
+                     l, nlo, nhi);
         lo = nlo;
         hi = nhi;
         best_err = lb_err;
@@ -687,21 +721,31 @@ __device__ __forceinline__ void encode_alpha_only(uint32_t px, bool valid, int l
     if (FMT == BC5) bc3_alpha((int)((px >> 8) & 255u), valid, l, out[2], out[3]);
 }
 
-// Encodes one 4x4 block per 16-lane group (lib.rs:188-234): lane l holds
-// pixel l (`px`, RGBA8 little-endian) and its mask bit (`valid`).  Every lane
-// of the wave must call this (LDS barriers inside); groups without a block
-// pass valid = false and discard the result.  Returns the block words in
-// out[0..3] (8-byte formats use out[0..1]).
+// The colour half of a BC1/BC2/BC3 block: words 0-1 for BC1, 2-3 otherwise.
+template <int FMT>
+__device__ __forceinline__ void set_colour(uint32_t (&out)[4], uint32_t lo, uint32_t hi)
+{
+    if (FMT == BC1) {
+        out[0] = lo;
+        out[1] = hi;
+    } else {
+        out[2] = lo;
+        out[3] = hi;
+    }
+}
+
+// Front half of compress_block_masked (lib.rs:188-230) for BC1/BC2/BC3,
+// one 4x4 block per 16-lane group (lane l holds pixel l, `px` RGBA8
+// little-endian, and its mask bit `valid`): alpha, colour set, fit choice,
+// and the finished colour half for the single-colour and range fits.
+// Returns the fit kind; for KIND_CLUSTER the principal axis comes back in
+// (vx, vy, vz) and the colour half is left to the caller.  Every lane of the
+// wave must call this (LDS hand-offs inside).
 template <int FMT, int ALG>
-__device__ __forceinline__ void encode_block(uint32_t px, bool valid, const FitParams& fp, GroupLds& L, int g, int l,
-                                             uint32_t (&out)[4])
+__device__ __forceinline__ int encode_front(uint32_t px, bool valid, const FitParams& fp, GroupLds& L, int g, int l,
+                                            uint32_t (&out)[4], ColourSet& cs, float& vx, float& vy, float& vz)
 {
     const int gshift = 16 * g;
-
-    if (FMT == BC4 || FMT == BC5) {
-        encode_alpha_only<FMT>(px, valid, l, out);
-        return;
-    }
     out[0] = out[1] = out[2] = out[3] = 0;
     if (FMT == BC2) bc2_alpha(px, valid, l, out[0], out[1]);
     if (FMT == BC3) bc3_alpha((int)(px >> 24), valid, l, out[0], out[1]);
@@ -710,7 +754,6 @@ __device__ __forceinline__ void encode_block(uint32_t px, bool valid, const FitP
     const uint32_t rgb = px & 0xFFFFFFu, alpha = px >> 24;
     const bool inc = valid && !(FMT == BC1 && alpha < 128u);
     const uint32_t incmask = (uint32_t)(__ballot(inc) >> gshift) & 0xFFFFu;
-    ColourSet cs;
     cs.transparent = FMT == BC1 && ((uint32_t)(__ballot(valid && alpha < 128u) >> gshift) & 0xFFFFu) != 0;
     L.pix[l] = px;
     group_sync();
@@ -742,9 +785,32 @@ __device__ __forceinline__ void encode_block(uint32_t px, bool valid, const FitP
                      : (ALG == RANGE_FIT || cs.n == 0) ? KIND_RANGE : KIND_CLUSTER;
     uint32_t lo = 0, hi = 0;
     if (kind == KIND_SINGLE) single_fit<FMT>(L, cs, l, lo, hi);
-    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    vx = 0.0f;
+    vy = 0.0f;
+    vz = 0.0f;
     if (kind != KIND_SINGLE) principal_axis(L, cs.n, vx, vy, vz);
     if (kind == KIND_RANGE) range_fit<FMT>(L, cs, l, fp, vx, vy, vz, lo, hi);
+    set_colour<FMT>(out, lo, hi);
+    return kind;
+}
+
+// Encodes one 4x4 block per 16-lane group (lib.rs:188-234): lane l holds
+// pixel l (`px`, RGBA8 little-endian) and its mask bit (`valid`).  Every lane
+// of the wave must call this (LDS hand-offs inside); groups without a block
+// pass valid = false and discard the result.  Returns the block words in
+// out[0..3] (8-byte formats use out[0..1]).  The four groups of a wave run
+// their ClusterFit searches in lockstep.
+template <int FMT, int ALG>
+__device__ __forceinline__ void encode_block(uint32_t px, bool valid, const FitParams& fp, GroupLds& L, int g, int l,
+                                             uint32_t (&out)[4])
+{
+    if (FMT == BC4 || FMT == BC5) {
+        encode_alpha_only<FMT>(px, valid, l, out);
+        return;
+    }
+    ColourSet cs;
+    float vx, vy, vz;
+    const int kind = encode_front<FMT, ALG>(px, valid, fp, L, g, l, out, cs, vx, vy, vz);
     if (ALG != RANGE_FIT) {
         // wave-uniform region: run_cluster contains LDS barriers
         constexpr int ITERS = ALG == ITERATIVE_CLUSTER_FIT ? 8 : 1;
@@ -755,18 +821,8 @@ __device__ __forceinline__ void encode_block(uint32_t px, bool valid, const FitP
             if (FMT == BC1) run_cluster<3, ITERS>(L, cs, l, act, vx, vy, vz, fp, best_err, clo, chi);
             run_cluster<4, ITERS>(L, cs, l, act && !(FMT == BC1 && cs.transparent), vx, vy, vz, fp, best_err, clo,
                                   chi);
-            if (act) {
-                lo = clo;
-                hi = chi;
-            }
+            if (act) set_colour<FMT>(out, clo, chi);
         }
-    }
-    if (FMT == BC1) {
-        out[0] = lo;
-        out[1] = hi;
-    } else {
-        out[2] = lo;
-        out[3] = hi;
     }
 }
 

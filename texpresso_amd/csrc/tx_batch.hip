@@ -193,6 +193,11 @@ struct tx_batch {
     hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint8_t* pipe_mem = nullptr;  // binned-pipeline buffers (tuning.pipeline)
+    txp::PipeBufs pipe;
+    bool any_bc1 = false;
+    uint32_t pipe_grid = 0;
+    uint64_t pipe_slots = 0;  // record slots covered by the heavy chunks
 };
 
 namespace {
@@ -217,8 +222,38 @@ uint32_t cost_class(const tx_texture& t)
 
 hipError_t launch_batch(tx_batch* b, hipStream_t s)
 {
-    return txk::launch_batch_raw(b->d_tex, b->d_chunks, b->nheavy, b->nchunks - b->nheavy, b->d_counter, b->grid,
-                                 b->grid_light, b->tuning.waves_per_group, s, b->light_stream, b->fork, b->join);
+    if (!b->tuning.pipeline || b->nheavy == 0)
+        return txk::launch_batch_raw(b->d_tex, b->d_chunks, b->nheavy, b->nchunks - b->nheavy, b->d_counter,
+                                     b->grid, b->grid_light, b->tuning.waves_per_group, s, b->light_stream, b->fork,
+                                     b->join);
+    // alpha-only chunks on the side stream, the binned pipeline on `s`
+    const uint32_t nlight = b->nchunks - b->nheavy;
+    hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, b->d_counter);
+    hipStream_t sl = s;
+    if (nlight && b->light_stream) {
+        if (hipEventRecord(b->fork, s) != hipSuccess || hipStreamWaitEvent(b->light_stream, b->fork, 0) != hipSuccess)
+            return hipErrorUnknown;
+        sl = b->light_stream;
+    }
+    if (nlight)
+        hipLaunchKernelGGL(txd::k_batch_light, dim3(b->grid_light), dim3(256), 0, sl, b->d_tex, b->d_chunks,
+                           b->nheavy, nlight, b->d_counter + 1);
+    txp::PipeLaunch p{};
+    p.d_tex = b->d_tex;
+    p.d_chunks = b->d_chunks;
+    p.nchunks = b->nheavy;
+    p.nslots = b->pipe_slots;
+    p.chunk_blocks = b->tuning.chunk_blocks;
+    p.classify_grid = std::min(b->pipe_grid, b->nheavy);
+    p.search_grid = b->pipe_grid;
+    p.any_bc1 = b->any_bc1;
+    const hipError_t e = txp::launch_pipeline(p, b->pipe, s);
+    if (e != hipSuccess) return e;
+    if (nlight && b->light_stream) {
+        if (hipEventRecord(b->join, b->light_stream) != hipSuccess || hipStreamWaitEvent(s, b->join, 0) != hipSuccess)
+            return hipErrorUnknown;
+    }
+    return hipGetLastError();
 }
 
 void free_batch(tx_batch* b)
@@ -226,6 +261,7 @@ void free_batch(tx_batch* b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->light_stream) (void)hipStreamDestroy(b->light_stream);
+    if (b->pipe_mem) (void)hipFree(b->pipe_mem);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->join) (void)hipEventDestroy(b->join);
     if (b->d_tex) (void)hipFree(b->d_tex);
@@ -245,6 +281,7 @@ tx_batch_tuning tx_batch_tuning_default(void)
     t.waves_per_group = 1;
     t.groups_per_cu = 12;
     t.use_graph = 1;
+    t.pipeline = 0;
     return t;
 }
 
@@ -257,7 +294,7 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
     tx_batch_tuning t = tuning ? *tuning : tx_batch_tuning_default();
     if (t.chunk_blocks == 0 || t.chunk_blocks % 4 != 0) return TX_ERR_INVALID;
     if (t.waves_per_group != 1 && t.waves_per_group != 2 && t.waves_per_group != 4) return TX_ERR_INVALID;
-    if (t.groups_per_cu == 0 || t.groups_per_cu > 64) return TX_ERR_INVALID;
+    if (t.groups_per_cu == 0 || t.groups_per_cu > 64 || t.pipeline > 1) return TX_ERR_INVALID;
     int rc = TX_OK;
     txc::DeviceContext* ctx = txc::current_context(&rc);
     if (!ctx) return rc;
@@ -298,8 +335,17 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
     std::vector<size_t> order(chunks.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cls[a] > cls[b]; });
+    // heavy chunks also get their first record slot of the binned pipeline
     std::vector<uint4> sorted(chunks.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = chunks[order[i]];
+    uint64_t slots = 0;
+    for (size_t i = 0; i < order.size(); ++i) {
+        sorted[i] = chunks[order[i]];
+        if (cls[order[i]] > 0) {
+            sorted[i].w = (uint32_t)slots;
+            slots += sorted[i].z;
+        }
+    }
+    if (slots > 0xFFFFFFFFull) return TX_ERR_INVALID;
 
     tx_batch* b = new (std::nothrow) tx_batch();
     if (!b) return TX_ERR_NOMEM;
@@ -329,6 +375,19 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
             free_batch(b);
             return TX_ERR_HIP;
         }
+    }
+    if (t.pipeline && b->nheavy) {
+        const uint64_t heavy_blocks = slots;
+        b->pipe_slots = slots;
+        for (size_t i = 0; i < count; ++i)
+            if (tex[i].variant == 1u || tex[i].variant == 2u) b->any_bc1 = true;
+        b->pipe_grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 12,
+                                                    std::max<uint64_t>(1, (heavy_blocks + 3) / 4));
+        if (hipMalloc(&b->pipe_mem, txp::PipeBufs::bytes_for(std::max<uint64_t>(1, heavy_blocks))) != hipSuccess) {
+            free_batch(b);
+            return TX_ERR_NOMEM;
+        }
+        b->pipe.carve(b->pipe_mem, std::max<uint64_t>(1, heavy_blocks));
     }
     if (hipMalloc(&b->d_tex, sizeof(txd::BatchTex) * std::max<size_t>(1, tex.size())) != hipSuccess ||
         hipMalloc(&b->d_chunks, sizeof(uint4) * std::max<size_t>(1, sorted.size())) != hipSuccess ||
@@ -418,7 +477,12 @@ int tx_batch_autotune(const tx_texture* textures, size_t count, const tx_batch_t
         const uint32_t chunk[] = {16, 32, 64, 128, 256};
         const uint32_t wpg[] = {1, 2, 4};
         const uint32_t waves_per_cu[] = {4, 8, 12};
-        for (uint32_t c : chunk)
+        for (uint32_t c : chunk) {
+            // binned pipeline: its grids are fixed, only the classify chunk varies
+            tx_batch_tuning p = tx_batch_tuning_default();
+            p.chunk_blocks = c;
+            p.pipeline = 1;
+            grid.push_back(p);
             for (uint32_t w : wpg)
                 for (uint32_t v : waves_per_cu) {
                     tx_batch_tuning t;
@@ -426,8 +490,10 @@ int tx_batch_autotune(const tx_texture* textures, size_t count, const tx_batch_t
                     t.waves_per_group = w;
                     t.groups_per_cu = std::max(1u, v / w);
                     t.use_graph = 1;
+                    t.pipeline = 0;
                     grid.push_back(t);
                 }
+        }
     }
     float best_t = -1.0f;
     for (const tx_batch_tuning& t : grid) {

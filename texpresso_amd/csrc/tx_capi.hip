@@ -5,6 +5,7 @@
 // device-buffer entry points only enqueue.  There is no CPU code path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string.h>
 #include <vector>
@@ -78,6 +79,13 @@ const char* tx_error_string(int code)
 
 void tx_single_lut(uint8_t out[6144]) { memcpy(out, &kHostLut, sizeof kHostLut); }
 
+uint64_t tx_set_pipeline_threshold(uint64_t min_blocks)
+{
+    const uint64_t old = txc::pipeline_min_blocks();
+    txc::set_pipeline_min_blocks(min_blocks);
+    return old;
+}
+
 const char* tx_device_arch(void)
 {
     txc::DeviceContext* ctx = txc::current_context(nullptr);
@@ -117,6 +125,48 @@ static txk::EncodeArgs encode_args(int format, const tx_params* p, const uint8_t
     return a;
 }
 
+// Encodes one texture on `s`: ClusterFit textures of at least
+// pipeline_min_blocks() blocks go through the binned pipeline (classify,
+// bin, independent per-block searches), everything else through the direct
+// kernel.  Both produce the same bytes.
+static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, hipStream_t s)
+{
+    const bool cluster = a.format <= TX_BC3 && a.algorithm != TX_RANGE_FIT;
+    if (!cluster || a.nblocks < txc::pipeline_min_blocks() || a.nblocks > 0xFFFFFFFFull)
+        return txk::launch_encode(a, s);
+    std::lock_guard<std::mutex> lock(ctx->pipe_mu);
+    txp::PipeBufs bufs;
+    if (!ctx->pipe_scratch(a.nblocks, &bufs)) return hipErrorOutOfMemory;
+    if (hipStreamWaitEvent(s, ctx->pipe_done, 0) != hipSuccess) return hipErrorUnknown;
+    txp::PipeLaunch p{};
+    p.d_tex = nullptr;
+    p.tex0.rgba = a.rgba;
+    p.tex0.out = a.out;
+    p.tex0.width = a.width;
+    p.tex0.height = a.height;
+    p.tex0.blocks_wide = a.blocks_wide;
+    p.tex0.nblocks = (uint32_t)a.nblocks;
+    p.tex0.variant = 3u * (uint32_t)a.format + (uint32_t)a.algorithm;
+    p.tex0.pad = 0;
+    p.tex0.wr = a.wr;
+    p.tex0.wg = a.wg;
+    p.tex0.wb = a.wb;
+    p.tex0.weigh_alpha = a.weigh_alpha;
+    const uint64_t waves = (uint64_t)ctx->cus * 12;
+    uint64_t cb = (a.nblocks + waves - 1) / waves;
+    cb = std::min<uint64_t>(64, std::max<uint64_t>(4, (cb + 3) & ~(uint64_t)3));
+    p.chunk_blocks = (uint32_t)cb;
+    p.nchunks = (uint32_t)((a.nblocks + cb - 1) / cb);
+    p.nslots = a.nblocks;
+    p.d_chunks = nullptr;
+    p.classify_grid = (uint32_t)std::min<uint64_t>(waves, p.nchunks);
+    p.search_grid = (uint32_t)std::min<uint64_t>(waves, (a.nblocks + 3) / 4);
+    p.any_bc1 = a.format == TX_BC1;
+    const hipError_t e = txp::launch_pipeline(p, bufs, s);
+    if (hipEventRecord(ctx->pipe_done, s) != hipSuccess) return hipErrorUnknown;
+    return e;
+}
+
 int tx_compress(int format, const uint8_t* rgba, size_t rgba_len, size_t width, size_t height,
                 const tx_params* params, uint8_t* output, size_t out_len)
 {
@@ -136,7 +186,7 @@ int tx_compress(int format, const uint8_t* rgba, size_t rgba_len, size_t width, 
     uint8_t* d_out = d + in_cap;
     if (in_bytes && hipMemcpyAsync(d_in, rgba, in_bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         return TX_ERR_HIP;
-    if (txk::launch_encode(encode_args(format, params, d_in, width, height, d_out, out_len), ctx->stream) !=
+    if (encode_on(ctx, encode_args(format, params, d_in, width, height, d_out, out_len), ctx->stream) !=
         hipSuccess)
         return TX_ERR_HIP;
     if (hipMemcpyAsync(output, d_out, out_len, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) return TX_ERR_HIP;
@@ -156,7 +206,7 @@ int tx_compress_device(int format, const uint8_t* d_rgba, size_t width, size_t h
     txc::DeviceContext* ctx = txc::current_context(&rc);
     if (!ctx) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    if (txk::launch_encode(encode_args(format, params, d_rgba, width, height, d_output, out_len), s) != hipSuccess)
+    if (encode_on(ctx, encode_args(format, params, d_rgba, width, height, d_output, out_len), s) != hipSuccess)
         return TX_ERR_HIP;
     return TX_OK;
 }

@@ -1,6 +1,7 @@
 // tx_context.hip -- per-device contexts of the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string.h>
 
 #include "texpresso_hip.h"
@@ -35,6 +36,35 @@ uint8_t* DeviceContext::scratch(size_t bytes)
     return pool;
 }
 
+bool DeviceContext::pipe_scratch(uint64_t recs, txp::PipeBufs* out)
+{
+    if (recs > pipe_recs || !pipe) {
+        uint64_t cap = pipe_recs ? pipe_recs : 4096;
+        while (cap < recs) cap *= 2;
+        if (pipe) {
+            (void)hipEventSynchronize(pipe_done);
+            (void)hipFree(pipe);
+            pipe = nullptr;
+            pipe_recs = 0;
+        }
+        if (hipMalloc(&pipe, txp::PipeBufs::bytes_for(cap)) != hipSuccess) {
+            pipe = nullptr;
+            return false;
+        }
+        pipe_recs = cap;
+    }
+    out->carve(pipe, pipe_recs);
+    return true;
+}
+
+namespace {
+std::atomic<uint64_t> g_pipe_min_blocks{131072};
+}
+
+uint64_t pipeline_min_blocks() { return g_pipe_min_blocks.load(); }
+
+void set_pipeline_min_blocks(uint64_t v) { g_pipe_min_blocks.store(v); }
+
 DeviceContext* current_context(int* err)
 {
     int dev = 0;
@@ -56,8 +86,10 @@ DeviceContext* current_context(int* err)
     }
     DeviceContext* c = new DeviceContext();
     c->device = dev;
+    c->cus = prop.multiProcessorCount;
     strncpy(c->arch, "gfx950", sizeof c->arch - 1);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->pipe_done, hipEventDisableTiming) != hipSuccess) {
         delete c;
         if (err) *err = TX_ERR_HIP;
         return nullptr;

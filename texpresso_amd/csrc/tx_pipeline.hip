@@ -1,0 +1,419 @@
+// tx_pipeline.hip -- binned ClusterFit pipeline: classify, bin, search.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "bcn_device.hpp"
+#include "tx_pipeline.hpp"
+
+#pragma clang fp contract(off)
+
+namespace txp {
+
+size_t PipeBufs::bytes_for(uint64_t cap)
+{
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    return up(sizeof(ClusterRec) * cap) + 3 * up(4 * cap) + up(sizeof(PipeCtl));
+}
+
+void PipeBufs::carve(uint8_t* base, uint64_t cap)
+{
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off = 0;
+    recs = reinterpret_cast<ClusterRec*>(base + off);
+    off += up(sizeof(ClusterRec) * cap);
+    keys = reinterpret_cast<uint32_t*>(base + off);
+    off += up(4 * cap);
+    list_a = reinterpret_cast<uint32_t*>(base + off);
+    off += up(4 * cap);
+    list_b = reinterpret_cast<uint32_t*>(base + off);
+    off += up(4 * cap);
+    ctl = reinterpret_cast<PipeCtl*>(base + off);
+    capacity = cap;
+}
+
+}  // namespace txp
+
+namespace txd {
+
+using txk::BatchTex;
+using txp::ClusterRec;
+using txp::PipeCtl;
+
+constexpr uint32_t kKeyA = 1u << 5, kKeyB = 1u << 6;
+
+__global__ void k_pipe_reset(PipeCtl* ctl)
+{
+    uint32_t* w = reinterpret_cast<uint32_t*>(ctl);
+    for (uint32_t i = threadIdx.x; i < sizeof(PipeCtl) / 4; i += blockDim.x) w[i] = 0u;
+}
+
+__device__ __forceinline__ const BatchTex& tex_at(const BatchTex* tex, const BatchTex& tex0, uint32_t i)
+{
+    return tex ? tex[i] : tex0;
+}
+
+// Per-workgroup bin counts of the classify pass, flushed once at its end.
+struct ClassifyHist {
+    uint32_t a[17], b[17];
+};
+
+// ClusterFit block: write its record into slot r and count it into the
+// workgroup's bins.
+template <int FMT, int ALG>
+__device__ __forceinline__ void write_record(ClusterRec* __restrict__ recs, uint32_t r, ClassifyHist& H,
+                                             const GroupLds& L, int l, const ColourSet& cs, const uint32_t (&out)[4],
+                                             float vx, float vy, float vz, const FitParams& fp, uint32_t tex_idx,
+                                             uint32_t block, uint32_t& key)
+{
+    ClusterRec& R = recs[r];
+    R.pts[l] = L.pts[l];
+    // one byte per pixel: word q holds pixels 4q..4q+3
+    const uint32_t byte = (uint32_t)(cs.remap < 0 ? 255 : cs.remap) << (8 * (l & 3));
+    const uint32_t r0 = grp_or((l >> 2) == 0 ? byte : 0u);
+    const uint32_t r1 = grp_or((l >> 2) == 1 ? byte : 0u);
+    const uint32_t r2 = grp_or((l >> 2) == 2 ? byte : 0u);
+    const uint32_t r3 = grp_or((l >> 2) == 3 ? byte : 0u);
+    const bool need4 = !(FMT == BC1 && cs.transparent);
+    if (l == 0) {
+        R.remap[0] = r0;
+        R.remap[1] = r1;
+        R.remap[2] = r2;
+        R.remap[3] = r3;
+        R.alpha[0] = out[0];
+        R.alpha[1] = out[1];
+        R.axis[0] = vx;
+        R.axis[1] = vy;
+        R.axis[2] = vz;
+        R.fit3_err = FLT_MAX;
+        R.fit3[0] = 0u;
+        R.fit3[1] = 0u;
+        R.w[0] = fp.wr;
+        R.w[1] = fp.wg;
+        R.w[2] = fp.wb;
+        R.tex = tex_idx;
+        R.block = block;
+        R.info = (uint32_t)cs.n | ((cs.transparent ? 1u : 0u) << 8) | ((uint32_t)FMT << 12) |
+                 ((ALG == ITERATIVE_CLUSTER_FIT ? 8u : 1u) << 16);
+        key = (uint32_t)cs.n | (FMT == BC1 ? kKeyA : 0u) | (need4 ? kKeyB : 0u);
+        if (FMT == BC1) atomicAdd(&H.a[cs.n], 1u);
+        if (need4) atomicAdd(&H.b[cs.n], 1u);
+    }
+}
+
+template <int FMT, int ALG>
+__device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_idx, uint32_t first, uint32_t count,
+                                               uint32_t slot0, GroupLds& L, int g, int l, ClusterRec* recs,
+                                               uint32_t* keys, ClassifyHist& H)
+{
+    const FitParams fp{T.wr, T.wg, T.wb, T.weigh_alpha};
+    for (uint32_t off = 0; off < count; off += 4) {
+        const uint32_t b = first + off + (uint32_t)g;
+        const bool live = off + (uint32_t)g < count;
+        const uint32_t bx = live ? b % T.blocks_wide : 0u;
+        const uint32_t by = live ? b / T.blocks_wide : 0u;
+        bool valid;
+        const uint32_t px = fetch_pixel(T.rgba, T.width, T.height, bx, by, live, l, valid);
+        uint32_t w[4];
+        ColourSet cs;
+        float vx, vy, vz;
+        const int kind = encode_front<FMT, ALG>(px, valid, fp, L, g, l, w, cs, vx, vy, vz);
+        const bool cluster = live && kind == KIND_CLUSTER;
+        if (live && !cluster && l == 0) {
+            if (FMT == BC1)
+                *reinterpret_cast<uint2*>(T.out + (size_t)b * 8u) = make_uint2(w[0], w[1]);
+            else
+                *reinterpret_cast<uint4*>(T.out + (size_t)b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        uint32_t key = 0;
+        if (cluster) write_record<FMT, ALG>(recs, slot0 + off + (uint32_t)g, H, L, l, cs, w, vx, vy, vz, fp, tex_idx, b,
+                                            key);
+        if (live && l == 0) keys[slot0 + off + (uint32_t)g] = key;
+    }
+}
+
+// Persistent classify over the heavy chunks (BC1/BC2/BC3 textures).  Chunk
+// c covers blocks [first, first + count) of one texture and record slots
+// [slot0, slot0 + count).
+__global__ __launch_bounds__(64) void k_pipe_classify(const BatchTex* __restrict__ tex, BatchTex tex0,
+                                                      const uint4* __restrict__ chunks, uint32_t nchunks,
+                                                      uint32_t chunk_blocks, ClusterRec* __restrict__ recs,
+                                                      uint32_t* __restrict__ keys, PipeCtl* __restrict__ ctl)
+{
+    __shared__ GroupLds lds[4];
+    __shared__ ClassifyHist H;
+    const int lane = (int)threadIdx.x, g = lane >> 4, l = lane & 15;
+    GroupLds& L = lds[g];
+    if (lane < 17) {
+        H.a[lane] = 0u;
+        H.b[lane] = 0u;
+    }
+    group_sync();
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(&ctl->chunk_counter, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= nchunks) break;
+        uint4 ch;
+        if (chunks) {
+            ch = chunks[c];
+        } else {
+            const uint32_t f = c * chunk_blocks;
+            ch = make_uint4(0u, f, min(chunk_blocks, tex0.nblocks - f), f);
+        }
+        const BatchTex& T = tex_at(tex, tex0, ch.x);
+        switch (T.variant) {
+        case 0: classify_chunk<BC1, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        case 1: classify_chunk<BC1, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        case 2: classify_chunk<BC1, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        case 3: classify_chunk<BC2, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        case 4: classify_chunk<BC2, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        case 5: classify_chunk<BC2, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        case 6: classify_chunk<BC3, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        case 7: classify_chunk<BC3, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        default: classify_chunk<BC3, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        }
+    }
+    // one flush of the workgroup's bin counts
+    group_sync();
+    if (lane < 17) {
+        const uint32_t a = H.a[lane], b = H.b[lane];
+        if (a) atomicAdd(&ctl->hist_a[lane], a);
+        if (b) atomicAdd(&ctl->hist_b[lane], b);
+    }
+}
+
+// Counting sort of the records by colour count, 16 first.  Each workgroup
+// takes kBinSeg consecutive slots: it counts its keys per bin in LDS,
+// reserves one range per non-empty bin with a single global atomic, then
+// places its records inside those ranges (order within a bin is free: it
+// only steers which records share a wave).
+constexpr uint32_t kBinSeg = 4096;
+
+__global__ __launch_bounds__(256) void k_pipe_bin(const uint32_t* __restrict__ keys, uint64_t nslots,
+                                                  PipeCtl* __restrict__ ctl, uint32_t* __restrict__ list_a,
+                                                  uint32_t* __restrict__ list_b)
+{
+    __shared__ uint32_t cnt_a[17], cnt_b[17], base_a[17], base_b[17];
+    const uint32_t t = threadIdx.x;
+    if (t < 17) {
+        cnt_a[t] = 0u;
+        cnt_b[t] = 0u;
+    }
+    __syncthreads();
+    const uint64_t s0 = (uint64_t)blockIdx.x * kBinSeg;
+    const uint64_t s1 = s0 + kBinSeg < nslots ? s0 + kBinSeg : nslots;
+    for (uint64_t r = s0 + t; r < s1; r += blockDim.x) {
+        const uint32_t k = keys[r];
+        if (k & kKeyA) atomicAdd(&cnt_a[k & 31u], 1u);
+        if (k & kKeyB) atomicAdd(&cnt_b[k & 31u], 1u);
+    }
+    __syncthreads();
+    if (t < 17) {
+        // bins are laid out 16 first: bin n starts after the counts of bins n+1..16
+        uint32_t oa = 0, ob = 0;
+        for (int m = 16; m > (int)t; --m) {
+            oa += ctl->hist_a[m];
+            ob += ctl->hist_b[m];
+        }
+        base_a[t] = cnt_a[t] ? oa + atomicAdd(&ctl->pos_a[t], cnt_a[t]) : 0u;
+        base_b[t] = cnt_b[t] ? ob + atomicAdd(&ctl->pos_b[t], cnt_b[t]) : 0u;
+        cnt_a[t] = 0u;
+        cnt_b[t] = 0u;
+    }
+    __syncthreads();
+    for (uint64_t r = s0 + t; r < s1; r += blockDim.x) {
+        const uint32_t k = keys[r];
+        const uint32_t n = k & 31u;
+        if (k & kKeyA) list_a[base_a[n] + atomicAdd(&cnt_a[n], 1u)] = (uint32_t)r;
+        if (k & kKeyB) list_b[base_b[n] + atomicAdd(&cnt_b[n], 1u)] = (uint32_t)r;
+    }
+}
+
+// Result of a finished block: the output block, or (NC = 3 for an opaque BC1
+// block) the 3-colour result kept in the record for the 4-colour pass.
+template <int NC>
+__device__ __forceinline__ void emit_block(const BatchTex* __restrict__ tex, const BatchTex& tex0, ClusterRec& R,
+                                           uint32_t info, const GroupLds& L, int l, int n, int remap, float lb_err,
+                                           float thr, uint32_t lb_code, int lb_it, const float (&sa)[3],
+                                           const float (&sb)[3])
+{
+    const uint32_t fmt = (info >> 12) & 15u;
+    uint32_t lo = 0, hi = 0;
+    const bool won = lb_err < thr;
+    if (won) {
+        write_colour(NC == 4, sa[0], sa[1], sa[2], sb[0], sb[1], sb[2], split_index<NC>(L, n, remap, lb_code, lb_it),
+                     l, lo, hi);
+    } else if (NC == 4 && fmt == BC1) {
+        lo = R.fit3[0];
+        hi = R.fit3[1];
+    }
+    if (l == 0) {
+        const BatchTex& T = tex_at(tex, tex0, R.tex);
+        if (NC == 3) {
+            if ((info >> 8) & 1u) {
+                *reinterpret_cast<uint2*>(T.out + (size_t)R.block * 8u) = make_uint2(lo, hi);
+            } else {
+                R.fit3_err = won ? lb_err : thr;
+                R.fit3[0] = lo;
+                R.fit3[1] = hi;
+            }
+        } else if (fmt == BC1) {
+            *reinterpret_cast<uint2*>(T.out + (size_t)R.block * 8u) = make_uint2(lo, hi);
+        } else {
+            *reinterpret_cast<uint4*>(T.out + (size_t)R.block * 16u) = make_uint4(R.alpha[0], R.alpha[1], lo, hi);
+        }
+    }
+}
+
+// Persistent search: each 16-lane group pulls records and runs the block's
+// ordering iterations independently (cluster.rs:174-249 / 301-389).  NC = 3:
+// BC1's compress3, whose result goes to the record (or, for a BC1 block with
+// transparent pixels, straight to the output).  NC = 4: compress4 from the
+// 3-colour result (BC1) or from nothing (BC2/BC3), then the output block.
+//
+// Each step has two phases.  Ordering phase: groups without a pending search
+// build their next ordering; an ordering equal to an earlier one ends that
+// block at once (cluster.rs:176-178, 303-305), and the group pulls its next record and
+// tries again, so no group enters the search phase empty-handed while records
+// remain.  Search phase: every group with a fresh ordering evaluates its
+// candidate splits; the candidate loop runs as long as the largest colour
+// count among them needs, and the records are sorted by colour count.
+template <int NC>
+__global__ __launch_bounds__(64) void k_pipe_search(const BatchTex* __restrict__ tex, BatchTex tex0,
+                                                    ClusterRec* __restrict__ recs, const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor)
+{
+    __shared__ GroupLds lds[4];
+    const int lane = (int)threadIdx.x, g = lane >> 4, l = lane & 15;
+    GroupLds& L = lds[g];
+    uint32_t total = 0;
+    for (int m = 0; m < 17; ++m) total += hist[m];
+
+    bool idle = false, need = true, ready = false;
+    uint32_t r = 0, info = 0, lb_code = 0;
+    int n = 0, iters = 1, it = 0, lb_it = 0, remap = -1;
+    float thr = FLT_MAX, lb_err = FLT_MAX;
+    float sa[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};
+    float axx = 0.0f, axy = 0.0f, axz = 0.0f;
+    FitParams fp{0.0f, 0.0f, 0.0f, 0u};
+    // the wave's private run of list positions [pool, pool_end), refilled
+    // kPull at a time with one global atomic
+    constexpr uint32_t kPull = 16;
+    uint32_t pool = 0, pool_end = 0;
+    for (;;) {
+        // ---- ordering phase
+        for (;;) {
+            // groups without a block take the next record
+            if (wave_any(need && !idle)) {
+                const bool take = need && !idle;
+                const uint64_t want = __ballot(take && l == 0);
+                const uint32_t nwant = (uint32_t)__popcll(want);
+                const uint32_t rank = (uint32_t)__popcll(want & ((1ull << (uint32_t)(g * 16)) - 1ull));
+                const uint32_t avail = pool_end - pool;
+                uint32_t p = pool + rank;
+                if (avail < nwant) {
+                    uint32_t fresh = 0;
+                    if (lane == 0) fresh = atomicAdd(cursor, kPull);
+                    fresh = __builtin_amdgcn_readfirstlane(fresh);
+                    if (rank >= avail) p = fresh + (rank - avail);
+                    pool = fresh + (nwant - avail);
+                    pool_end = fresh + kPull;
+                } else {
+                    pool += nwant;
+                }
+                if (take) {
+                    if (p >= total) {
+                        idle = true;
+                    } else {
+                        r = list[p];
+                        const ClusterRec& R = recs[r];
+                        L.pts[l] = R.pts[l];
+                        info = R.info;
+                        n = (int)(info & 31u);
+                        iters = (int)((info >> 16) & 15u);
+                        const uint32_t byte = (R.remap[l >> 2] >> (8 * (l & 3))) & 255u;
+                        remap = byte == 255u ? -1 : (int)byte;
+                        fp = FitParams{R.w[0], R.w[1], R.w[2], 0u};
+                        axx = R.axis[0];
+                        axy = R.axis[1];
+                        axz = R.axis[2];
+                        thr = NC == 4 ? R.fit3_err : FLT_MAX;
+                        lb_err = thr;
+                        lb_code = 0;
+                        lb_it = 0;
+                        it = 0;
+                        sa[0] = sa[1] = sa[2] = 0.0f;
+                        sb[0] = sb[1] = sb[2] = 0.0f;
+                        need = false;
+                    }
+                }
+                group_sync();
+            }
+            const bool todo = !idle && !ready;
+            if (!wave_any(todo)) break;
+            const bool unique = ordering_step(L, l, n, todo, it, axx, axy, axz);
+            ready = ready || unique;
+            const bool repeated = todo && !unique;
+            if (wave_any(repeated)) {
+                if (repeated) {
+                    emit_block<NC>(tex, tex0, recs[r], info, L, l, n, remap, lb_err, thr, lb_code, lb_it, sa, sb);
+                    need = true;
+                }
+            }
+        }
+        if (!wave_any(ready)) break;
+        // ---- search phase
+        float be;
+        uint32_t bc;
+        search_step<NC>(L, l, n, ready, fp, be, bc);
+        bool fin = false;
+        if (ready) {
+            ready = false;
+            if (be < lb_err) {
+                lb_err = be;
+                lb_code = bc;
+                lb_it = it;
+                split_endpoints<NC>(L, n, bc, fp, sa, sb);
+            }
+            if (lb_it != it || it + 1 >= iters) {
+                fin = true;
+            } else {
+                axx = sb[0] - sa[0];
+                axy = sb[1] - sa[1];
+                axz = sb[2] - sa[2];
+                ++it;
+            }
+        }
+        if (wave_any(fin)) {
+            if (fin) {
+                emit_block<NC>(tex, tex0, recs[r], info, L, l, n, remap, lb_err, thr, lb_code, lb_it, sa, sb);
+                need = true;
+            }
+        }
+    }
+}
+
+}  // namespace txd
+
+namespace txp {
+
+hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s)
+{
+    if (p.nslots > b.capacity || p.nslots > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(txd::k_pipe_reset, dim3(1), dim3(64), 0, s, b.ctl);
+    if (p.nchunks == 0 || p.nslots == 0) return hipGetLastError();
+    hipLaunchKernelGGL(txd::k_pipe_classify, dim3(p.classify_grid), dim3(64), 0, s, p.d_tex, p.tex0, p.d_chunks,
+                       p.nchunks, p.chunk_blocks, b.recs, b.keys, b.ctl);
+    const uint32_t bin_grid = (uint32_t)((p.nslots + txd::kBinSeg - 1) / txd::kBinSeg);
+    hipLaunchKernelGGL(txd::k_pipe_bin, dim3(bin_grid), dim3(256), 0, s, b.keys, p.nslots, b.ctl, b.list_a,
+                       b.list_b);
+    if (p.any_bc1)
+        hipLaunchKernelGGL(txd::k_pipe_search<3>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
+                           b.list_a, b.ctl->hist_a, &b.ctl->cursor_a);
+    hipLaunchKernelGGL(txd::k_pipe_search<4>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs, b.list_b,
+                       b.ctl->hist_b, &b.ctl->cursor_b);
+    return hipGetLastError();
+}
+
+}  // namespace txp

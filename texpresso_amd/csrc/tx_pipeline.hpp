@@ -1,0 +1,79 @@
+// tx_pipeline.hpp -- the binned ClusterFit pipeline (host-side interface).
+//
+//   reset -> classify -> bin -> search<3> (BC1) -> search<4>
+//
+// classify: alpha, colour set and fit choice for every block; single-colour
+//   and range-fit blocks are finished here, ClusterFit blocks become a
+//   ClusterRec and are counted by distinct-colour count n.
+// bin: counting sort of the records by n (16 first) into one list per pass
+//   (list A: BC1 records for the 3-colour search; list B: records that need
+//   the 4-colour search).
+// search<NC>: persistent waves; each 16-lane group pulls the next record
+//   from its list and runs that block's ordering iterations on its own, so
+//   blocks with one ordering and blocks with several share a wave without
+//   waiting for each other, and neighbouring records have the same n (the
+//   same number of candidate rounds).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bcn_kernels.hpp"
+
+namespace txp {
+
+// One ClusterFit block between the passes (336 bytes).
+struct __align__(16) ClusterRec {
+    float4 pts[16];     // distinct colours: x, y, z, sqrt(weight)
+    uint32_t remap[4];  // one byte per pixel: point index, 255 = excluded
+    uint32_t alpha[2];  // alpha half of the block (BC2/BC3)
+    float axis[3];      // principal axis
+    float fit3_err;     // 3-colour result: ClusterFit::best_error after compress3
+    uint32_t fit3[2];   // 3-colour result: ClusterFit::best_compressed after compress3
+    float w[3];         // Params::weights
+    uint32_t tex;       // texture index
+    uint32_t block;     // block index within the texture
+    uint32_t info;      // n | transparent << 8 | format << 12 | iterations << 16
+    uint32_t pad[2];
+};
+static_assert(sizeof(ClusterRec) == 336, "record layout");
+
+// Counters of one pipeline run (cleared by the reset kernel).  Every update
+// is aggregated (per workgroup or per batch of records) so that no counter
+// takes one global atomic per block.
+struct PipeCtl {
+    uint32_t chunk_counter;
+    uint32_t cursor_a, cursor_b;
+    uint32_t hist_a[17], hist_b[17];
+    uint32_t pos_a[17], pos_b[17];
+};
+
+struct PipeBufs {
+    ClusterRec* recs = nullptr;  // capacity records; slot = the block's position among the chunks
+    uint32_t* keys = nullptr;    // capacity: n | in list A << 5 | in list B << 6; 0 = no record
+    uint32_t* list_a = nullptr;  // capacity
+    uint32_t* list_b = nullptr;  // capacity
+    PipeCtl* ctl = nullptr;
+    uint64_t capacity = 0;
+
+    // bytes of device memory for `cap` records (one allocation, 256-aligned parts)
+    static size_t bytes_for(uint64_t cap);
+    // carves the parts out of one allocation of bytes_for(cap) bytes
+    void carve(uint8_t* base, uint64_t cap);
+};
+
+struct PipeLaunch {
+    const txk::BatchTex* d_tex;  // texture descriptors (null: only `tex0`)
+    txk::BatchTex tex0;          // the texture when d_tex is null
+    const uint4* d_chunks;       // heavy chunks (texture, first block, count, first record slot);
+                                 // null: implicit chunks of tex0
+    uint32_t nchunks;            // heavy chunks
+    uint32_t chunk_blocks;       // implicit chunk size
+    uint64_t nslots;             // record slots the chunks cover (<= capacity)
+    uint32_t classify_grid;      // persistent 64-thread workgroups of classify
+    uint32_t search_grid;        // persistent 64-thread workgroups of each search
+    bool any_bc1;                // run the 3-colour search
+};
+
+hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s);
+
+}  // namespace txp

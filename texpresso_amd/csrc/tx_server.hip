@@ -345,6 +345,7 @@ tx_server_config tx_server_config_default(void)
     c.max_pending = 4096;
     c.tuning = tx_batch_tuning_default();
     c.tuning.use_graph = 0;
+    c.tuning.pipeline = 0;  // launches use the single persistent kernel
     return c;
 }
 
