@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BC1/BC3 ClusterFit Mpix/s on 1 MI355X (bit-exact) vs host CPU; p99 job latency"
 VALU_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (FMA = 2 flops, packed)
-VALU_PLAIN_TOPS = 39.3       # one FP32 op per lane per clock: 256 CU x 64 lanes x 2.4 GHz
+VALU_PLAIN_TOPS = 39.3       # one plain FP32 op (mul/add/fma) per lane per clock: 256 CU x 64 lanes x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
 
 
@@ -40,6 +40,31 @@ def split_counts(n):
 
 C3 = np.array([split_counts(n)[0] for n in range(17)], np.float64)
 C4 = np.array([split_counts(n)[1] for n in range(17)], np.float64)
+
+
+def search_flops(stats):
+    """The part of algorithmic_flops spent in the split searches (per-split
+    costs plus the per-ordering work), split by pass: (3-colour, 4-colour)."""
+    n = stats["count"].clip(0, 16)
+    cl = stats["fit"] == 2
+    f3 = np.where(cl, C3[n] * 138 * stats["searched3"] + 450 * stats["searched3"], 0.0)
+    f4 = np.where(cl, C4[n] * 159 * stats["searched4"] + 450 * stats["searched4"], 0.0)
+    return float(f3.sum()), float(f4.sum())
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 summary (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = [x.strip() for x in line.strip().strip("|").split("|")]
+                if len(parts) >= 5 and parts[0] == kernel and parts[1] == workload:
+                    return int(float(parts[4]))
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def algorithmic_flops(stats):
@@ -442,6 +467,8 @@ def main():
     mpix = W * H / 1e6
     value = mpix / (ms / 1e3)
     gpu_out = d_out.download()
+    # per-kernel split of the same launches (events between the pipeline's kernels)
+    phases = tx.time_phases(fmt, d_in.ptr, W, H, params, d_out.ptr, out_len, args.steps)
 
     # BC5 on the normal map (second half of the workload), same protocol
     n5 = tx.Format.Bc5.compressed_size(W, H)
@@ -470,6 +497,8 @@ def main():
         "wall_s": round(wall, 4),
         "secondary": {"bc5_normal_map_mpix_s": round(mpix / (ms5 / 1e3), 1), "bc5_ms_per_step": round(ms5, 4)},
     }
+    if phases is not None:
+        line["secondary"]["pipeline_ms_per_step"] = {k: round(v / args.steps, 4) for k, v in phases.items()}
 
     import ctypes
     probe = ctypes.c_double()
@@ -484,12 +513,22 @@ def main():
         bad = int(np.count_nonzero(np.any(gpu_out.reshape(-1, 16) != want.reshape(-1, 16), axis=1)))
         line["parity"] = {"blocks": int(want.size // 16), "mismatched_blocks": bad}
         flops = algorithmic_flops(stats)
-        achieved = flops / (ms / 1e3) / 1e12
+        workload = line["config"]["workload"]
+        if phases is not None:
+            # dominant kernel: the 4-colour split search of the binned pipeline
+            f3, f4 = search_flops(stats)
+            kernel, k_ms, k_flops = "k_pipe_search<4>", phases["search4"] / args.steps, f4
+        else:
+            kernel, k_ms, k_flops = "k_encode<BC3,IterativeClusterFit>", ms, flops
+        achieved = k_flops / (k_ms / 1e3) / 1e12
         line["roofline"] = {
-            "bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / VALU_PEAK_TFLOPS, 4), "traffic": None,
-            "plain_fp32_peak_tops": VALU_PLAIN_TOPS, "frac_of_plain": round(achieved / VALU_PLAIN_TOPS, 4),
-            "flops_per_launch": flops,
+            "bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK_TFLOPS, 4),
+            "traffic": profiled_traffic(kernel, workload), "kernel_ms": round(k_ms, 4),
+            "algorithmic_flops_per_launch": k_flops, "path_flops_per_step": flops,
+            "path_tflops": round(flops / (ms / 1e3) / 1e12, 3),
+            "plain_fp32_peak_tops": VALU_PLAIN_TOPS,
+            "algorithmic_bytes_per_step": 4 * W * H + out_len,
             "hbm_gbs": round((4 * W * H + out_len) / (ms / 1e3) / 1e9, 2), "hbm_peak_gbs": HBM_PEAK_GBS,
         }
         # CPU baseline: the oracle on the same texture over `threads` host cores

@@ -229,6 +229,14 @@ int tx_util_copy(void* d_dst, const void* d_src, size_t bytes);
  * library stream, waits, and stores the elapsed milliseconds. */
 int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
                           uint8_t* d_output, size_t out_len, int iters, float* ms_total);
+/* Per-phase timing of the binned ClusterFit pipeline: runs tx_compress_device
+ * `iters` times on the library stream with events between its kernels and
+ * adds up, in ms_phase[0..3], classify, bin, 3-colour search and 4-colour
+ * search milliseconds.  *used_pipeline = 0 (and nothing is timed) when the
+ * texture goes through the direct kernel (below tx_set_pipeline_threshold,
+ * RangeFit, BC4/BC5). */
+int tx_util_time_phases(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
+                        uint8_t* d_output, size_t out_len, int iters, float ms_phase[4], int* used_pipeline);
 
 /* ClusterFit textures with at least `min_blocks` blocks are encoded by the
  * binned pipeline (classify, counting sort by colour count, per-block

@@ -168,6 +168,8 @@ def lib():
             "tx_set_pipeline_threshold": (ctypes.c_uint64, [ctypes.c_uint64]),
             "tx_util_time_compress": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
                                               ctypes.POINTER(ctypes.c_float)]),
+            "tx_util_time_phases": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
+                                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(c_int)]),
             "tx_batch_tuning_default": (_TxTuning, []),
             "tx_batch_create": (c_int, [ctypes.POINTER(c_vp), ctypes.POINTER(_TxTexture), c_sz,
                                         ctypes.POINTER(_TxTuning)]),
@@ -353,6 +355,20 @@ def time_compress(fmt, d_rgba, width, height, params, d_out, out_len, iters):
     _check(lib().tx_util_time_compress(int(fmt), d_rgba, width, height, ctypes.byref(p), d_out, out_len, iters,
                                        ctypes.byref(ms)), "time_compress")
     return ms.value
+
+
+def time_phases(fmt, d_rgba, width, height, params, d_out, out_len, iters):
+    """Per-phase milliseconds of the binned ClusterFit pipeline summed over
+    `iters` launches: {"classify", "bin", "search3", "search4"}, or None when
+    the texture goes through the direct kernel."""
+    ms = (ctypes.c_float * 4)()
+    used = ctypes.c_int()
+    p = params._c()
+    _check(lib().tx_util_time_phases(int(fmt), d_rgba, width, height, ctypes.byref(p), d_out, out_len, iters, ms,
+                                     ctypes.byref(used)), "time_phases")
+    if not used.value:
+        return None
+    return dict(zip(("classify", "bin", "search3", "search4"), (ms[0], ms[1], ms[2], ms[3])))
 
 
 class Tuning:

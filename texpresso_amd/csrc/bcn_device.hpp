@@ -55,6 +55,7 @@ struct __align__(16) GroupLds {
 // objects are built without relocatable device code).
 static __constant__ txt::SingleLut c_lut = txt::make_single_lut();
 static __constant__ txt::CandTables c_cand = txt::make_cand_tables();
+static __constant__ txt::UnitTables c_unit = txt::make_unit_tables();
 
 // ---------------------------------------------------------------- helpers
 
@@ -452,10 +453,13 @@ __device__ __forceinline__ SplitSums load_split(const GroupLds& L, uint32_t code
 
 // One split of the ClusterFit search (cluster.rs:189-230 for compress3,
 // 320-364 for compress4).  Returns the weighted error; the quantised
-// endpoints come back through a and b.
-template <int NC>
+// endpoints come back through a and b.  With UNIT the weight terms
+// (alpha2_sum, beta2_sum, alphabeta_sum, factor) come from `u`, the
+// unit-weight table entry of this candidate, which holds exactly the values
+// the expressions below produce when every point weight is 1.
+template <int NC, bool UNIT = false>
 __device__ __forceinline__ float eval_split(const SplitSums& s, const float4& xs, const FitParams& fp, float (&a)[3],
-                                            float (&b)[3])
+                                            float (&b)[3], const txt::UnitSplit& u = txt::UnitSplit{})
 {
     const float4 p0 = s.p0;
     const float4 p1 = s.p1;
@@ -466,31 +470,46 @@ __device__ __forceinline__ float eval_split(const SplitSums& s, const float4& xs
         const float p3x = ((xs.x - p2.x) - p1.x) - p0.x;
         const float p3y = ((xs.y - p2.y) - p1.y) - p0.y;
         const float p3z = ((xs.z - p2.z) - p1.z) - p0.z;
-        const float p3w = ((xs.w - p2.w) - p1.w) - p0.w;
         const float t13 = 1.0f / 3.0f, t23 = 2.0f / 3.0f, n19 = 1.0f / 9.0f, n49 = 4.0f / 9.0f;
         // alphax = part2*(1/3,1/3,1/3,1/9) + (part1*(2/3,2/3,2/3,4/9) + part0)
         ax[0] = p2.x * t13 + (p1.x * t23 + p0.x);
         ax[1] = p2.y * t13 + (p1.y * t23 + p0.y);
         ax[2] = p2.z * t13 + (p1.z * t23 + p0.z);
-        a2 = p2.w * n19 + (p1.w * n49 + p0.w);
         // betax = part1*(1/3,..,1/9) + (part2*(2/3,..,4/9) + part3)
         bx[0] = p1.x * t13 + (p2.x * t23 + p3x);
         bx[1] = p1.y * t13 + (p2.y * t23 + p3y);
         bx[2] = p1.z * t13 + (p2.z * t23 + p3z);
-        b2 = p1.w * n19 + (p2.w * n49 + p3w);
-        ab = (2.0f / 9.0f) * (p1.w + p2.w);
+        if (UNIT) {
+            a2 = u.a2;
+            b2 = u.b2;
+            ab = u.ab;
+        } else {
+            const float p3w = ((xs.w - p2.w) - p1.w) - p0.w;
+            a2 = p2.w * n19 + (p1.w * n49 + p0.w);
+            b2 = p1.w * n19 + (p2.w * n49 + p3w);
+            ab = (2.0f / 9.0f) * (p1.w + p2.w);
+        }
     } else {
         // part2 = xsum - part1 - part0; h = part1*(.5,.5,.5,.25)
         const float p2x = (xs.x - p1.x) - p0.x;
         const float p2y = (xs.y - p1.y) - p0.y;
         const float p2z = (xs.z - p1.z) - p0.z;
-        const float p2w = (xs.w - p1.w) - p0.w;
-        const float hx = p1.x * 0.5f, hy = p1.y * 0.5f, hz = p1.z * 0.5f, hw = p1.w * 0.25f;
-        ax[0] = hx + p0.x; ax[1] = hy + p0.y; ax[2] = hz + p0.z; a2 = hw + p0.w;
-        bx[0] = hx + p2x; bx[1] = hy + p2y; bx[2] = hz + p2z; b2 = hw + p2w;
-        ab = hw;
+        const float hx = p1.x * 0.5f, hy = p1.y * 0.5f, hz = p1.z * 0.5f;
+        ax[0] = hx + p0.x; ax[1] = hy + p0.y; ax[2] = hz + p0.z;
+        bx[0] = hx + p2x; bx[1] = hy + p2y; bx[2] = hz + p2z;
+        if (UNIT) {
+            a2 = u.a2;
+            b2 = u.b2;
+            ab = u.ab;
+        } else {
+            const float p2w = (xs.w - p1.w) - p0.w;
+            const float hw = p1.w * 0.25f;
+            a2 = hw + p0.w;
+            b2 = hw + p2w;
+            ab = hw;
+        }
     }
-    const float factor = 1.0f / ((a2 * b2) - ab * ab);
+    const float factor = UNIT ? u.factor : 1.0f / ((a2 * b2) - ab * ab);
     const float grid[3] = {31.0f, 63.0f, 31.0f};
     const float rcp[3] = {1.0f / 31.0f, 1.0f / 63.0f, 1.0f / 31.0f};
     const float wts[3] = {fp.wr, fp.wg, fp.wb};
@@ -584,30 +603,30 @@ __device__ __forceinline__ bool ordering_step(GroupLds& L, int l, int n, bool ac
     return unique;
 }
 
-// The split search over the current prefix table: every group with `run`
-// set evaluates its `len` candidates (16 lanes, list entries two rounds and
-// prefix sums one round ahead) and returns the first strict minimum
-// (error, code) on all of its lanes.  Executed by the whole wave.
-template <int NC>
-__device__ __forceinline__ void search_step(const GroupLds& L, int l, int n, bool run, const FitParams& fp,
-                                            float& be, uint32_t& bc)
+// Candidate loop of one split search (see search_step).
+template <int NC, bool UNIT>
+__device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t mylen, const float4& xs,
+                                            const FitParams& fp, float& be, uint32_t& bc)
 {
-    const uint32_t mylen = run ? (NC == 4 ? c_cand.len4[n] : c_cand.len3[n]) : 0u;
     const uint32_t* tab = NC == 4 ? c_cand.t4 : c_cand.t3;
-    const float4 xs = L.S[n];  // xsum_wsum = S[0][n]
+    const txt::UnitSplit* ut = NC == 4 ? c_unit.u4 : c_unit.u3;
     be = __builtin_huge_valf();
     bc = 0xFFFFFFFFu;
     const uint32_t t0 = (uint32_t)l;
     uint32_t c0 = t0 < mylen ? tab[t0] : 0u;
     uint32_t c1 = t0 + 16 < mylen ? tab[t0 + 16] : 0u;
     SplitSums s0 = load_split<NC>(L, c0);
+    txt::UnitSplit u0{};
+    if (UNIT && t0 < mylen) u0 = ut[t0];
     for (uint32_t base = 0; wave_any(base < mylen); base += 16) {
         const uint32_t t = base + (uint32_t)l;
         const uint32_t c2 = t + 32 < mylen ? tab[t + 32] : 0u;
         const SplitSums s1 = load_split<NC>(L, c1);
+        txt::UnitSplit u1{};
+        if (UNIT && t + 16 < mylen) u1 = ut[t + 16];
         if (t < mylen) {
             float a[3], b[3];
-            const float err = eval_split<NC>(s0, xs, fp, a, b);
+            const float err = eval_split<NC, UNIT>(s0, xs, fp, a, b, u0);
             if (err < be || (err == be && c0 < bc)) {
                 be = err;
                 bc = c0;
@@ -616,7 +635,27 @@ __device__ __forceinline__ void search_step(const GroupLds& L, int l, int n, boo
         c0 = c1;
         c1 = c2;
         s0 = s1;
+        u0 = u1;
     }
+}
+
+// The split search over the current prefix table: every group with `run`
+// set evaluates its `len` candidates (16 lanes, list entries two rounds and
+// prefix sums one round ahead) and returns the first strict minimum
+// (error, code) on all of its lanes.  Executed by the whole wave.  When
+// every searching group holds 16 points of weight exactly 1, the weight
+// terms come from the unit-weight table (same values, fewer operations).
+template <int NC>
+__device__ __forceinline__ void search_step(const GroupLds& L, int l, int n, bool run, const FitParams& fp,
+                                            float& be, uint32_t& bc)
+{
+    const uint32_t mylen = run ? (NC == 4 ? c_cand.len4[n] : c_cand.len3[n]) : 0u;
+    const float4 xs = L.S[n];  // xsum_wsum = S[0][n]
+    const bool unit = n == 16 && grp_min(L.pts[l].w == 1.0f ? 1u : 0u) == 1u;
+    if (!wave_any(run && !unit))
+        search_loop<NC, true>(L, l, mylen, xs, fp, be, bc);
+    else
+        search_loop<NC, false>(L, l, mylen, xs, fp, be, bc);
     grp_argmin(be, bc);
 }
 

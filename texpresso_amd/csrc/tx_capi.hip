@@ -128,12 +128,17 @@ static txk::EncodeArgs encode_args(int format, const tx_params* p, const uint8_t
 // Encodes one texture on `s`: ClusterFit textures of at least
 // pipeline_min_blocks() blocks go through the binned pipeline (classify,
 // bin, independent per-block searches), everything else through the direct
-// kernel.  Both produce the same bytes.
-static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, hipStream_t s)
+// kernel.  Both produce the same bytes.  With `marks` (5 events) the
+// pipeline records them between its kernels; `used_pipeline` says which
+// path ran.
+static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, hipStream_t s,
+                            hipEvent_t* marks = nullptr, bool* used_pipeline = nullptr)
 {
     const bool cluster = a.format <= TX_BC3 && a.algorithm != TX_RANGE_FIT;
+    if (used_pipeline) *used_pipeline = false;
     if (!cluster || a.nblocks < txc::pipeline_min_blocks() || a.nblocks > 0xFFFFFFFFull)
         return txk::launch_encode(a, s);
+    if (used_pipeline) *used_pipeline = true;
     std::lock_guard<std::mutex> lock(ctx->pipe_mu);
     txp::PipeBufs bufs;
     if (!ctx->pipe_scratch(a.nblocks, &bufs)) return hipErrorOutOfMemory;
@@ -162,7 +167,7 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
     p.classify_grid = (uint32_t)std::min<uint64_t>(waves, p.nchunks);
     p.search_grid = (uint32_t)std::min<uint64_t>(waves, (a.nblocks + 3) / 4);
     p.any_bc1 = a.format == TX_BC1;
-    const hipError_t e = txp::launch_pipeline(p, bufs, s);
+    const hipError_t e = txp::launch_pipeline(p, bufs, s, marks);
     if (hipEventRecord(ctx->pipe_done, s) != hipSuccess) return hipErrorUnknown;
     return e;
 }
@@ -371,6 +376,39 @@ int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_
     if (rc == TX_OK && hipEventElapsedTime(ms_total, e0, e1) != hipSuccess) rc = TX_ERR_HIP;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int tx_util_time_phases(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
+                        uint8_t* d_output, size_t out_len, int iters, float ms_phase[4], int* used_pipeline)
+{
+    if (!ms_phase || !used_pipeline || iters < 1) return TX_ERR_INVALID;
+    size_t in_bytes = 0;
+    int rc = check_compress(format, width, height, params, out_len, &in_bytes);
+    if (rc != TX_OK) return rc;
+    if (!d_rgba || !d_output) return TX_ERR_INVALID;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    hipEvent_t ev[5] = {};
+    for (int i = 0; i < 5 && rc == TX_OK; ++i)
+        if (hipEventCreate(&ev[i]) != hipSuccess) rc = TX_ERR_HIP;
+    for (int k = 0; k < 4; ++k) ms_phase[k] = 0.0f;
+    const txk::EncodeArgs a = encode_args(format, params, d_rgba, width, height, d_output, out_len);
+    for (int it = 0; it < iters && rc == TX_OK; ++it) {
+        bool pipe = false;
+        if (encode_on(ctx, a, ctx->stream, ev, &pipe) != hipSuccess) rc = TX_ERR_HIP;
+        *used_pipeline = pipe ? 1 : 0;
+        if (rc != TX_OK || !pipe) break;
+        if (hipEventSynchronize(ev[4]) != hipSuccess) rc = TX_ERR_HIP;
+        for (int k = 0; k < 4 && rc == TX_OK; ++k) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) rc = TX_ERR_HIP;
+            ms_phase[k] += ms;
+        }
+    }
+    if (rc == TX_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = TX_ERR_HIP;
+    for (int i = 0; i < 5; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
     return rc;
 }
 

@@ -398,21 +398,31 @@ __global__ __launch_bounds__(64) void k_pipe_search(const BatchTex* __restrict__
 
 namespace txp {
 
-hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s)
+hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s, hipEvent_t* marks)
 {
     if (p.nslots > b.capacity || p.nslots > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    auto mark = [&](int i) { return marks ? hipEventRecord(marks[i], s) : hipSuccess; };
     hipLaunchKernelGGL(txd::k_pipe_reset, dim3(1), dim3(64), 0, s, b.ctl);
-    if (p.nchunks == 0 || p.nslots == 0) return hipGetLastError();
+    if (p.nchunks == 0 || p.nslots == 0) {
+        for (int i = 0; i < 5; ++i)
+            if (mark(i) != hipSuccess) return hipErrorUnknown;
+        return hipGetLastError();
+    }
+    if (mark(0) != hipSuccess) return hipErrorUnknown;
     hipLaunchKernelGGL(txd::k_pipe_classify, dim3(p.classify_grid), dim3(64), 0, s, p.d_tex, p.tex0, p.d_chunks,
                        p.nchunks, p.chunk_blocks, b.recs, b.keys, b.ctl);
+    if (mark(1) != hipSuccess) return hipErrorUnknown;
     const uint32_t bin_grid = (uint32_t)((p.nslots + txd::kBinSeg - 1) / txd::kBinSeg);
     hipLaunchKernelGGL(txd::k_pipe_bin, dim3(bin_grid), dim3(256), 0, s, b.keys, p.nslots, b.ctl, b.list_a,
                        b.list_b);
+    if (mark(2) != hipSuccess) return hipErrorUnknown;
     if (p.any_bc1)
         hipLaunchKernelGGL(txd::k_pipe_search<3>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
                            b.list_a, b.ctl->hist_a, &b.ctl->cursor_a);
+    if (mark(3) != hipSuccess) return hipErrorUnknown;
     hipLaunchKernelGGL(txd::k_pipe_search<4>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs, b.list_b,
                        b.ctl->hist_b, &b.ctl->cursor_b);
+    if (mark(4) != hipSuccess) return hipErrorUnknown;
     return hipGetLastError();
 }
 

@@ -74,6 +74,10 @@ struct PipeLaunch {
     bool any_bc1;                // run the 3-colour search
 };
 
-hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s);
+// Launches reset, classify, bin and the searches on `s`.  With `marks`
+// (5 events), marks[0] is recorded before classify, marks[1] after it,
+// marks[2] after bin, marks[3] after the 3-colour search (recorded even when
+// there is none) and marks[4] after the 4-colour search.
+hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s, hipEvent_t* marks = nullptr);
 
 }  // namespace txp

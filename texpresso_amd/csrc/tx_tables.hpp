@@ -153,4 +153,67 @@ constexpr CandTables make_cand_tables()
     return T;
 }
 
+// 3. Unit-weight split constants.  When a block has 16 distinct colours and
+//    every point weight is exactly 1 (no duplicates, no alpha weighting),
+//    the weight components of the split sums are exact integers fixed by
+//    the split alone: part0.w = i, part1.w = j - i, part2.w = k - j,
+//    xsum.w = 16.  alpha2_sum, beta2_sum, alphabeta_sum and the factor of
+//    cluster.rs:193-202 / 325-335 are then the same numbers for every such
+//    block.  They are evaluated here with the kernel's own expressions
+//    (constant evaluation is IEEE binary32 with round-to-nearest-even and no
+//    contraction, like the device code), one entry per candidate in the
+//    order of CandTables, so the device reads bit-identical values instead
+//    of recomputing them.
+struct UnitSplit {
+    float a2, b2, ab, factor;
+};
+
+struct UnitTables {
+    UnitSplit u4[kCand4];
+    UnitSplit u3[kCand3];
+};
+
+constexpr UnitSplit unit_split4(int i, int j, int k)
+{
+    const float p0w = (float)i, p1w = (float)(j - i), p2w = (float)(k - j), xsw = 16.0f;
+    const float p3w = ((xsw - p2w) - p1w) - p0w;
+    const float n19 = 1.0f / 9.0f, n49 = 4.0f / 9.0f;
+    const float a2 = p2w * n19 + (p1w * n49 + p0w);
+    const float b2 = p1w * n19 + (p2w * n49 + p3w);
+    const float ab = (2.0f / 9.0f) * (p1w + p2w);
+    return UnitSplit{a2, b2, ab, 1.0f / ((a2 * b2) - ab * ab)};
+}
+
+constexpr UnitSplit unit_split3(int i, int j)
+{
+    const float p0w = (float)i, p1w = (float)(j - i), xsw = 16.0f;
+    const float p2w = (xsw - p1w) - p0w;
+    const float hw = p1w * 0.25f;
+    const float a2 = hw + p0w, b2 = hw + p2w, ab = hw;
+    return UnitSplit{a2, b2, ab, 1.0f / ((a2 * b2) - ab * ab)};
+}
+
+// (i, j, k) of a packed candidate: i from bits 16..19, j and k from the
+// prefix-table offsets tri(i, j) and tri(j, k).
+constexpr int cand_j(uint32_t code, int i) { return (int)(code & 255u) - tri(i, 0); }
+
+constexpr UnitTables make_unit_tables()
+{
+    const CandTables C = make_cand_tables();
+    UnitTables U{};
+    for (int e = 0; e < kCand4; ++e) {
+        const uint32_t code = C.t4[e];
+        const int i = (int)((code >> 16) & 15u);
+        const int j = cand_j(code, i);
+        const int k = (int)((code >> 8) & 255u) - tri(j, 0);
+        U.u4[e] = unit_split4(i, j, k);
+    }
+    for (int e = 0; e < kCand3; ++e) {
+        const uint32_t code = C.t3[e];
+        const int i = (int)((code >> 16) & 15u);
+        U.u3[e] = unit_split3(i, cand_j(code, i));
+    }
+    return U;
+}
+
 }  // namespace txt
