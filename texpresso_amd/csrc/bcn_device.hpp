@@ -612,30 +612,44 @@ __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t m
     const txt::UnitSplit* ut = NC == 4 ? c_unit.u4 : c_unit.u3;
     be = __builtin_huge_valf();
     bc = 0xFFFFFFFFu;
+    // Two candidates per lane and trip (A at t, B at t + 16).  Codes are read
+    // two half-trips ahead and prefix sums one half-trip ahead; writing each
+    // carried value back in the trip that consumed it keeps it in the same
+    // registers (no copies at the back edge).  The kept minimum is the
+    // smallest (error, code), so evaluation order does not matter.
     const uint32_t t0 = (uint32_t)l;
-    uint32_t c0 = t0 < mylen ? tab[t0] : 0u;
-    uint32_t c1 = t0 + 16 < mylen ? tab[t0 + 16] : 0u;
-    SplitSums s0 = load_split<NC>(L, c0);
-    txt::UnitSplit u0{};
-    if (UNIT && t0 < mylen) u0 = ut[t0];
-    for (uint32_t base = 0; wave_any(base < mylen); base += 16) {
+    uint32_t cA = t0 < mylen ? tab[t0] : 0u;
+    uint32_t cB = t0 + 16 < mylen ? tab[t0 + 16] : 0u;
+    SplitSums sA = load_split<NC>(L, cA);
+    txt::UnitSplit uA{};
+    if (UNIT && t0 < mylen) uA = ut[t0];
+    for (uint32_t base = 0; wave_any(base < mylen); base += 32) {
         const uint32_t t = base + (uint32_t)l;
-        const uint32_t c2 = t + 32 < mylen ? tab[t + 32] : 0u;
-        const SplitSums s1 = load_split<NC>(L, c1);
-        txt::UnitSplit u1{};
-        if (UNIT && t + 16 < mylen) u1 = ut[t + 16];
+        const uint32_t cA2 = t + 32 < mylen ? tab[t + 32] : 0u;
+        const SplitSums sB = load_split<NC>(L, cB);
+        txt::UnitSplit uB{};
+        if (UNIT && t + 16 < mylen) uB = ut[t + 16];
         if (t < mylen) {
             float a[3], b[3];
-            const float err = eval_split<NC, UNIT>(s0, xs, fp, a, b, u0);
-            if (err < be || (err == be && c0 < bc)) {
+            const float err = eval_split<NC, UNIT>(sA, xs, fp, a, b, uA);
+            if (err < be || (err == be && cA < bc)) {
                 be = err;
-                bc = c0;
+                bc = cA;
             }
         }
-        c0 = c1;
-        c1 = c2;
-        s0 = s1;
-        u0 = u1;
+        const uint32_t cB2 = t + 48 < mylen ? tab[t + 48] : 0u;
+        sA = load_split<NC>(L, cA2);
+        if (UNIT && t + 32 < mylen) uA = ut[t + 32];
+        if (t + 16 < mylen) {
+            float a[3], b[3];
+            const float err = eval_split<NC, UNIT>(sB, xs, fp, a, b, uB);
+            if (err < be || (err == be && cB < bc)) {
+                be = err;
+                bc = cB;
+            }
+        }
+        cA = cA2;
+        cB = cB2;
     }
 }
 
