@@ -41,6 +41,14 @@ struct FitParams {
     uint32_t weigh_alpha;  // Params::weigh_colour_by_alpha
 };
 
+// LDS of the front half of a block (colour set, principal axis, single and
+// range fits): the pixels and the distinct colours, 320 bytes.  The front
+// functions take any LDS type with these two members.
+struct __align__(16) FrontLds {
+    float4 pts[16];    // distinct colours: x, y, z, sqrt(weight)
+    uint32_t pix[16];  // the block's RGBA8 pixels
+};
+
 // Per-group LDS (one per 4x4 block in flight), 3.2 KiB.
 struct __align__(16) GroupLds {
     float4 S[txt::kTriEntries];  // prefix sums of the weighted ordered points
@@ -264,7 +272,8 @@ struct ColourSet {
 
 // lib/src/math.rs:44-97: weighted covariance of the n points in LDS, then
 // eight power-iteration steps.  Folds run in point order on every lane.
-__device__ __forceinline__ void principal_axis(const GroupLds& L, int n, float& vx, float& vy, float& vz)
+template <class LDS>
+__device__ __forceinline__ void principal_axis(const LDS& L, int n, float& vx, float& vy, float& vz)
 {
     float total = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 #pragma unroll
@@ -332,8 +341,8 @@ __device__ __forceinline__ uint32_t single_endpoints(int tr, int tg, int cr, int
 }
 
 // single.rs:100-164
-template <int FMT>
-__device__ __forceinline__ void single_fit(const GroupLds& L, const ColourSet& cs, int l, uint32_t& lo, uint32_t& hi)
+template <int FMT, class LDS>
+__device__ __forceinline__ void single_fit(const LDS& L, const ColourSet& cs, int l, uint32_t& lo, uint32_t& hi)
 {
     const float4 q = L.pts[0];
     const int cr = f32_to_i32_clamped(q.x * 255.0f, 255);
@@ -363,8 +372,8 @@ __device__ __forceinline__ float snap(float v, float grid, float rcp)
 }
 
 // range.rs:44-192
-template <int FMT>
-__device__ __forceinline__ void range_fit(const GroupLds& L, const ColourSet& cs, int l, const FitParams& fp,
+template <int FMT, class LDS>
+__device__ __forceinline__ void range_fit(const LDS& L, const ColourSet& cs, int l, const FitParams& fp,
                                           float px, float py, float pz, uint32_t& lo, uint32_t& hi)
 {
     const int n = cs.n;
@@ -794,8 +803,8 @@ __device__ __forceinline__ void set_colour(uint32_t (&out)[4], uint32_t lo, uint
 // Returns the fit kind; for KIND_CLUSTER the principal axis comes back in
 // (vx, vy, vz) and the colour half is left to the caller.  Every lane of the
 // wave must call this (LDS hand-offs inside).
-template <int FMT, int ALG>
-__device__ __forceinline__ int encode_front(uint32_t px, bool valid, const FitParams& fp, GroupLds& L, int g, int l,
+template <int FMT, int ALG, class LDS>
+__device__ __forceinline__ int encode_front(uint32_t px, bool valid, const FitParams& fp, LDS& L, int g, int l,
                                             uint32_t (&out)[4], ColourSet& cs, float& vx, float& vy, float& vz)
 {
     const int gshift = 16 * g;

@@ -198,6 +198,7 @@ struct tx_batch {
     bool any_bc1 = false;
     uint32_t pipe_grid = 0;
     uint64_t pipe_slots = 0;  // record slots covered by the heavy chunks
+    uint32_t classify_grid = 0;
 };
 
 namespace {
@@ -244,7 +245,7 @@ hipError_t launch_batch(tx_batch* b, hipStream_t s)
     p.nchunks = b->nheavy;
     p.nslots = b->pipe_slots;
     p.chunk_blocks = b->tuning.chunk_blocks;
-    p.classify_grid = std::min(b->pipe_grid, b->nheavy);
+    p.classify_grid = std::min(b->classify_grid, b->nheavy);
     p.search_grid = b->pipe_grid;
     p.any_bc1 = b->any_bc1;
     const hipError_t e = txp::launch_pipeline(p, b->pipe, s);
@@ -381,8 +382,9 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
         b->pipe_slots = slots;
         for (size_t i = 0; i < count; ++i)
             if (tex[i].variant == 1u || tex[i].variant == 2u) b->any_bc1 = true;
-        b->pipe_grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 12,
+        b->pipe_grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * txp::kSearchWavesPerCu,
                                                     std::max<uint64_t>(1, (heavy_blocks + 3) / 4));
+        b->classify_grid = (uint32_t)prop.multiProcessorCount * txp::kClassifyWavesPerCu;
         if (hipMalloc(&b->pipe_mem, txp::PipeBufs::bytes_for(std::max<uint64_t>(1, heavy_blocks))) != hipSuccess) {
             free_batch(b);
             return TX_ERR_NOMEM;

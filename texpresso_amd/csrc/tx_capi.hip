@@ -157,15 +157,18 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
     p.tex0.wg = a.wg;
     p.tex0.wb = a.wb;
     p.tex0.weigh_alpha = a.weigh_alpha;
-    const uint64_t waves = (uint64_t)ctx->cus * 12;
-    uint64_t cb = (a.nblocks + waves - 1) / waves;
+    // persistent single-wave workgroups: classify fits 20 per CU (320 B of
+    // LDS per block), the searches 12 (3.2 KiB per block)
+    const uint64_t cwaves = (uint64_t)ctx->cus * txp::kClassifyWavesPerCu;
+    const uint64_t swaves = (uint64_t)ctx->cus * txp::kSearchWavesPerCu;
+    uint64_t cb = (a.nblocks + cwaves - 1) / cwaves;
     cb = std::min<uint64_t>(64, std::max<uint64_t>(4, (cb + 3) & ~(uint64_t)3));
     p.chunk_blocks = (uint32_t)cb;
     p.nchunks = (uint32_t)((a.nblocks + cb - 1) / cb);
     p.nslots = a.nblocks;
     p.d_chunks = nullptr;
-    p.classify_grid = (uint32_t)std::min<uint64_t>(waves, p.nchunks);
-    p.search_grid = (uint32_t)std::min<uint64_t>(waves, (a.nblocks + 3) / 4);
+    p.classify_grid = (uint32_t)std::min<uint64_t>(cwaves, p.nchunks);
+    p.search_grid = (uint32_t)std::min<uint64_t>(swaves, (a.nblocks + 3) / 4);
     p.any_bc1 = a.format == TX_BC1;
     const hipError_t e = txp::launch_pipeline(p, bufs, s, marks);
     if (hipEventRecord(ctx->pipe_done, s) != hipSuccess) return hipErrorUnknown;

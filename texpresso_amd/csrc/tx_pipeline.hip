@@ -59,10 +59,11 @@ struct ClassifyHist {
 };
 
 // ClusterFit block: write its record into slot r and count it into the
-// workgroup's bins.
+// workgroup's bins.  Lane l stores point l; lanes 0-4 store the 80-byte
+// tail (every value there is the same on all lanes of the group).
 template <int FMT, int ALG>
 __device__ __forceinline__ void write_record(ClusterRec* __restrict__ recs, uint32_t r, ClassifyHist& H,
-                                             const GroupLds& L, int l, const ColourSet& cs, const uint32_t (&out)[4],
+                                             const FrontLds& L, int l, const ColourSet& cs, const uint32_t (&out)[4],
                                              float vx, float vy, float vz, const FitParams& fp, uint32_t tex_idx,
                                              uint32_t block, uint32_t& key)
 {
@@ -75,45 +76,46 @@ __device__ __forceinline__ void write_record(ClusterRec* __restrict__ recs, uint
     const uint32_t r2 = grp_or((l >> 2) == 2 ? byte : 0u);
     const uint32_t r3 = grp_or((l >> 2) == 3 ? byte : 0u);
     const bool need4 = !(FMT == BC1 && cs.transparent);
+    const uint32_t info = (uint32_t)cs.n | ((cs.transparent ? 1u : 0u) << 8) | ((uint32_t)FMT << 12) |
+                          ((ALG == ITERATIVE_CLUSTER_FIT ? 8u : 1u) << 16);
+    uint4 v;
+    switch (l) {
+    case 0: v = make_uint4(r0, r1, r2, r3); break;
+    case 1: v = make_uint4(out[0], out[1], __float_as_uint(vx), __float_as_uint(vy)); break;
+    case 2: v = make_uint4(__float_as_uint(vz), __float_as_uint(FLT_MAX), 0u, 0u); break;
+    case 3: v = make_uint4(__float_as_uint(fp.wr), __float_as_uint(fp.wg), __float_as_uint(fp.wb), tex_idx); break;
+    default: v = make_uint4(block, info, 0u, 0u); break;
+    }
+    if (l < 5) reinterpret_cast<uint4*>(R.remap)[l] = v;
     if (l == 0) {
-        R.remap[0] = r0;
-        R.remap[1] = r1;
-        R.remap[2] = r2;
-        R.remap[3] = r3;
-        R.alpha[0] = out[0];
-        R.alpha[1] = out[1];
-        R.axis[0] = vx;
-        R.axis[1] = vy;
-        R.axis[2] = vz;
-        R.fit3_err = FLT_MAX;
-        R.fit3[0] = 0u;
-        R.fit3[1] = 0u;
-        R.w[0] = fp.wr;
-        R.w[1] = fp.wg;
-        R.w[2] = fp.wb;
-        R.tex = tex_idx;
-        R.block = block;
-        R.info = (uint32_t)cs.n | ((cs.transparent ? 1u : 0u) << 8) | ((uint32_t)FMT << 12) |
-                 ((ALG == ITERATIVE_CLUSTER_FIT ? 8u : 1u) << 16);
         key = (uint32_t)cs.n | (FMT == BC1 ? kKeyA : 0u) | (need4 ? kKeyB : 0u);
         if (FMT == BC1) atomicAdd(&H.a[cs.n], 1u);
         if (need4) atomicAdd(&H.b[cs.n], 1u);
     }
 }
 
+// Blocks [first, first + count) of one texture, four at a time; the next
+// four blocks' pixels are loaded before the current four are classified.
 template <int FMT, int ALG>
 __device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_idx, uint32_t first, uint32_t count,
-                                               uint32_t slot0, GroupLds& L, int g, int l, ClusterRec* recs,
+                                               uint32_t slot0, FrontLds& L, int g, int l, ClusterRec* recs,
                                                uint32_t* keys, ClassifyHist& H)
 {
     const FitParams fp{T.wr, T.wg, T.wb, T.weigh_alpha};
+    auto fetch = [&](uint32_t off, bool& valid) {
+        const uint32_t b = first + off + (uint32_t)g;
+        const bool live = off + (uint32_t)g < count;
+        return fetch_pixel(T.rgba, T.width, T.height, live ? b % T.blocks_wide : 0u, live ? b / T.blocks_wide : 0u,
+                           live, l, valid);
+    };
+    bool v_next;
+    uint32_t px_next = fetch(0, v_next);
     for (uint32_t off = 0; off < count; off += 4) {
         const uint32_t b = first + off + (uint32_t)g;
         const bool live = off + (uint32_t)g < count;
-        const uint32_t bx = live ? b % T.blocks_wide : 0u;
-        const uint32_t by = live ? b / T.blocks_wide : 0u;
-        bool valid;
-        const uint32_t px = fetch_pixel(T.rgba, T.width, T.height, bx, by, live, l, valid);
+        const bool valid = v_next;
+        const uint32_t px = px_next;
+        px_next = fetch(off + 4, v_next);
         uint32_t w[4];
         ColourSet cs;
         float vx, vy, vz;
@@ -140,10 +142,10 @@ __global__ __launch_bounds__(64) void k_pipe_classify(const BatchTex* __restrict
                                                       uint32_t chunk_blocks, ClusterRec* __restrict__ recs,
                                                       uint32_t* __restrict__ keys, PipeCtl* __restrict__ ctl)
 {
-    __shared__ GroupLds lds[4];
+    __shared__ FrontLds lds[4];
     __shared__ ClassifyHist H;
     const int lane = (int)threadIdx.x, g = lane >> 4, l = lane & 15;
-    GroupLds& L = lds[g];
+    FrontLds& L = lds[g];
     if (lane < 17) {
         H.a[lane] = 0u;
         H.b[lane] = 0u;

@@ -15,6 +15,7 @@
 //   same number of candidate rounds).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "bcn_kernels.hpp"
@@ -36,6 +37,13 @@ struct __align__(16) ClusterRec {
     uint32_t pad[2];
 };
 static_assert(sizeof(ClusterRec) == 336, "record layout");
+// the classify pass stores the tail as five 16-byte words starting at remap
+static_assert(offsetof(ClusterRec, remap) == 256 && offsetof(ClusterRec, alpha) == 272 &&
+                  offsetof(ClusterRec, axis) == 280 && offsetof(ClusterRec, fit3_err) == 292 &&
+                  offsetof(ClusterRec, fit3) == 296 && offsetof(ClusterRec, w) == 304 &&
+                  offsetof(ClusterRec, tex) == 316 && offsetof(ClusterRec, block) == 320 &&
+                  offsetof(ClusterRec, info) == 324,
+              "record tail layout");
 
 // Counters of one pipeline run (cleared by the reset kernel).  Every update
 // is aggregated (per workgroup or per batch of records) so that no counter
@@ -60,6 +68,11 @@ struct PipeBufs {
     // carves the parts out of one allocation of bytes_for(cap) bytes
     void carve(uint8_t* base, uint64_t cap);
 };
+
+// Resident single-wave workgroups per CU: classify is limited by registers
+// (5 waves per SIMD), the searches by LDS (4 x 3.2 KiB per workgroup).
+constexpr uint32_t kClassifyWavesPerCu = 20;
+constexpr uint32_t kSearchWavesPerCu = 12;
 
 struct PipeLaunch {
     const txk::BatchTex* d_tex;  // texture descriptors (null: only `tex0`)
