@@ -148,8 +148,9 @@ def suite_cfg0(tx, texgen, O, threads, steps):
             "valu_tflops": round(flops / (ms / 1e3) / 1e12, 2)}
 
 
-def suite_cfg2(tx, texgen, O, threads, steps):
-    """full mip chain of a seeded 4096x4096 texture, per format."""
+def suite_cfg2(tx, texgen, O, threads, steps, tuning_path):
+    """configs[2]: full mip chain of a seeded 4096x4096 texture, per format;
+    the chain is one batch with the autotuned tuning for that format."""
     seed = texgen.SEED_BASE + 2
     base = texgen.with_alpha_field(texgen.rgb_texture(4096, 4096, seed), seed)
     chain = texgen.mip_chain(base)
@@ -162,7 +163,8 @@ def suite_cfg2(tx, texgen, O, threads, steps):
         p = tx.Params(alg)
         items = [(fmt, p, l) for l in chain]
         bufs, descs = _upload_set(tx, items)
-        batch = tx.Batch(descs)
+        best, _ = tx.tuned("cfg2-%s-%s" % (name, alg.name), descs, table_path=tuning_path, retune=True)
+        batch = tx.Batch(descs, best)
         batch.run_timed(1)
         ms_batch = batch.run_timed(steps) / steps
         got = _download(bufs)
@@ -184,7 +186,7 @@ def suite_cfg2(tx, texgen, O, threads, steps):
                      "per_level_mpix_s": round(px / 1e6 / (ms_plain / 1e3), 1),
                      "cpu_mpix_s": round(px / 1e6 / t_cpu, 3), "cpu_threads": threads,
                      "gpu_over_cpu": round((px / ms_batch / 1e3) / (px / t_cpu / 1e6), 1),
-                     "mismatched_bytes": bad, "block_bytes": bs,
+                     "mismatched_bytes": bad, "block_bytes": bs, "tuning": best.as_dict(),
                      "valu_tflops": round(flops / (ms_batch / 1e3) / 1e12, 3),
                      "valu_frac_of_157": round(flops / (ms_batch / 1e3) / 1e12 / VALU_PEAK_TFLOPS, 4),
                      "hbm_gbs": round(traffic / (ms_batch / 1e3) / 1e9, 1),
@@ -390,7 +392,7 @@ def run_suite(args):
         out.append(suite_cfg0(tx, texgen, O, threads, args.steps))
         print(json.dumps(out[-1]), flush=True)
     if "cfg2" in which:
-        for r in suite_cfg2(tx, texgen, O, threads, args.steps):
+        for r in suite_cfg2(tx, texgen, O, threads, args.steps, args.tuning_table):
             out.append(r)
             print(json.dumps(r), flush=True)
     if "cfg3" in which:
@@ -542,10 +544,17 @@ def main():
             if times[-1] > 30:
                 break
         t_cpu = float(np.median(times))
+        # one core on a bounded band of the same texture (the first 256 rows)
+        band = min(H, 256)
+        t0 = time.perf_counter()
+        O.compress(int(fmt), img[:band], W, band, op, threads=1)
+        t_one = time.perf_counter() - t0
         line["cpu_baseline"] = {
             "value": round(W * 4 * sample_rows / 1e6 / t_cpu, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
             "sample": "full %dx%d BC3 IterativeClusterFit texture, median of %d runs, %d threads on %s"
                       % (W, 4 * sample_rows, len(times), threads, cpu_model()),
+            "single_core_mpix_s": round(W * band / 1e6 / t_one, 3),
+            "single_core_sample": "rows 0-%d of the same texture, 1 thread" % (band - 1),
         }
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
     print(json.dumps(line), flush=True)
