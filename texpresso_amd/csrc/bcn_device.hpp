@@ -764,6 +764,113 @@ __device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, in
     }
 }
 
+// alpha.rs:187-256 for a whole block on one lane (BC4 / BC5 channels): the
+// same arithmetic as bc3_alpha, with the 16 pixels of the block in `v` and
+// their mask bits in `valid`.
+__device__ __forceinline__ void bc3_alpha_lane(const int (&v)[16], uint32_t valid, uint32_t& lo, uint32_t& hi)
+{
+    int min7 = 255, max7 = 0, min5 = 255, max5 = 0;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        if ((valid >> p) & 1u) {
+            min7 = min(min7, v[p]);
+            max7 = max(max7, v[p]);
+            if (v[p] != 0) min5 = min(min5, v[p]);
+            if (v[p] != 255) max5 = max(max5, v[p]);
+        }
+    }
+    if (min5 > max5) min5 = max5;
+    if (min7 > max7) min7 = max7;
+    fix_range(min5, max5, 5);
+    fix_range(min7, max7, 7);
+    int codes5[8], codes7[8];
+    codes5[0] = min5;
+    codes5[1] = max5;
+#pragma unroll
+    for (int i = 1; i < 5; ++i) codes5[1 + i] = ((5 - i) * min5 + i * max5) / 5;
+    codes5[6] = 0;
+    codes5[7] = 255;
+    codes7[0] = min5;
+    codes7[1] = max5;
+#pragma unroll
+    for (int i = 1; i < 7; ++i) codes7[1 + i] = ((7 - i) * min7 + i * max7) / 7;
+    // 3-bit indices of both codebooks, pixel p at bits 3p; masked pixels 0
+    uint64_t i5 = 0, i7 = 0;
+    uint32_t err5 = 0, err7 = 0;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        if ((valid >> p) & 1u) {
+            uint32_t l5, l7;
+            i5 |= (uint64_t)fit8(v[p], codes5, l5) << (3 * p);
+            i7 |= (uint64_t)fit8(v[p], codes7, l7) << (3 * p);
+            err5 += l5;
+            err7 += l7;
+        }
+    }
+    int a0, a1;
+    uint64_t idx = 0;
+    if (err5 <= err7) {  // write_alpha_block5, alpha.rs:146-165
+        if (min5 > max5) {
+            a0 = max5; a1 = min5;
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const uint32_t i = (uint32_t)(i5 >> (3 * p)) & 7u;
+                const uint32_t o = i == 0 ? 1u : i == 1 ? 0u : (i >= 2 && i <= 5) ? 7u - i : i;
+                idx |= (uint64_t)o << (3 * p);
+            }
+        } else {
+            a0 = min5; a1 = max5; idx = i5;
+        }
+    } else {  // write_alpha_block7, alpha.rs:167-185
+        if (min7 < max7) {
+            a0 = max7; a1 = min7;
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const uint32_t i = (uint32_t)(i7 >> (3 * p)) & 7u;
+                const uint32_t o = i == 0 ? 1u : i == 1 ? 0u : 9u - i;
+                idx |= (uint64_t)o << (3 * p);
+            }
+        } else {
+            a0 = min7; a1 = max7; idx = i7;
+        }
+    }
+    // alpha.rs:120-144: the 48 index bits follow the two endpoint bytes
+    lo = (uint32_t)a0 | ((uint32_t)a1 << 8) | ((uint32_t)(idx & 0xFFFFu) << 16);
+    hi = (uint32_t)(idx >> 16);
+}
+
+// One block per lane: the 16 pixels of block (bx, by) as 16 dword loads
+// (pixels outside the image are masked out, lib.rs:309-330).
+__device__ __forceinline__ uint32_t fetch_block_lane(const uint8_t* __restrict__ rgba, uint32_t width, uint32_t height,
+                                                     uint32_t bx, uint32_t by, uint32_t (&px)[16])
+{
+    uint32_t valid = 0;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const uint32_t sx = bx * 4u + (uint32_t)(p & 3), sy = by * 4u + (uint32_t)(p >> 2);
+        const bool ok = sx < width && sy < height;
+        px[p] = ok ? *reinterpret_cast<const uint32_t*>(rgba + 4 * ((size_t)sy * width + sx)) : 0u;
+        valid |= (ok ? 1u : 0u) << p;
+    }
+    return valid;
+}
+
+// BC4 (channel 0) or BC5 (channels 0 and 1) block from one lane's pixels.
+template <int FMT>
+__device__ __forceinline__ void encode_alpha_lane(const uint32_t (&px)[16], uint32_t valid, uint32_t (&out)[4])
+{
+    int v[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) v[p] = (int)(px[p] & 255u);
+    bc3_alpha_lane(v, valid, out[0], out[1]);
+    out[2] = out[3] = 0u;
+    if (FMT == BC5) {
+#pragma unroll
+        for (int p = 0; p < 16; ++p) v[p] = (int)((px[p] >> 8) & 255u);
+        bc3_alpha_lane(v, valid, out[2], out[3]);
+    }
+}
+
 // lib.rs:309-330: lane l of the group fetches pixel l of block (bx, by);
 // pixels outside the image are masked out.
 __device__ __forceinline__ uint32_t fetch_pixel(const uint8_t* __restrict__ rgba, uint32_t width, uint32_t height,

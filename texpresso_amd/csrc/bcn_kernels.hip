@@ -38,6 +38,24 @@ __global__ __launch_bounds__(64) void k_encode(const uint8_t* __restrict__ rgba,
     if (live && l == 0) store_block<FMT>(out, b, w);
 }
 
+// BC4 / BC5 per-texture launch: one block per lane (alpha codebooks are
+// integer work on 16 values; a lane's 16 dword loads keep many bytes in
+// flight, and neighbouring lanes read neighbouring 16-byte row segments).
+template <int FMT>
+__global__ __launch_bounds__(256) void k_encode_alpha(const uint8_t* __restrict__ rgba, uint32_t width,
+                                                      uint32_t height, uint32_t blocks_wide, uint64_t nblocks,
+                                                      uint8_t* __restrict__ out)
+{
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    uint32_t px[16];
+    const uint32_t valid = fetch_block_lane(rgba, width, height, (uint32_t)(b % blocks_wide),
+                                            (uint32_t)(b / blocks_wide), px);
+    uint32_t w[4];
+    encode_alpha_lane<FMT>(px, valid, w);
+    store_block<FMT>(out, b, w);
+}
+
 // Explicit block list (Format::compress_block_masked over many blocks): block
 // b's 16 pixels are the 64 bytes at blocks + 64*b, its mask is masks[b].
 template <int FMT, int ALG>
@@ -194,14 +212,25 @@ hipError_t launch_decode_t(const uint8_t* data, uint32_t w, uint32_t h, uint8_t*
 
 namespace txk {
 
+template <int FMT>
+hipError_t launch_encode_alpha(const EncodeArgs& a, hipStream_t s)
+{
+    const uint64_t groups = (a.nblocks + 255) / 256;
+    if (groups == 0) return hipSuccess;
+    if (groups > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((txd::k_encode_alpha<FMT>), dim3((uint32_t)groups), dim3(256), 0, s, a.rgba, a.width, a.height,
+                       a.blocks_wide, a.nblocks, a.out);
+    return hipGetLastError();
+}
+
 hipError_t launch_encode(const EncodeArgs& a, hipStream_t s)
 {
     switch (a.format) {
     case 0: return launch_encode_f<0>(a, s);
     case 1: return launch_encode_f<1>(a, s);
     case 2: return launch_encode_f<2>(a, s);
-    case 3: return launch_encode_t<3, 0>(a, s);
-    case 4: return launch_encode_t<4, 0>(a, s);
+    case 3: return launch_encode_alpha<3>(a, s);
+    case 4: return launch_encode_alpha<4>(a, s);
     default: return hipErrorInvalidValue;
     }
 }

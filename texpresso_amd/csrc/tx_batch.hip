@@ -76,37 +76,34 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch(const BatchTex* __restrict
     }
 }
 
-__device__ __forceinline__ uint32_t chunk_pixel(const BatchTex& T, uint32_t first, uint32_t count, uint32_t off, int g,
-                                                int l, bool& valid)
-{
-    const uint32_t b = first + off + (uint32_t)g;
-    const bool live = off + (uint32_t)g < count;
-    const uint32_t bx = live ? b % T.blocks_wide : 0u;
-    const uint32_t by = live ? b / T.blocks_wide : 0u;
-    return fetch_pixel(T.rgba, T.width, T.height, bx, by, live, l, valid);
-}
-
-// Memory-bound: the next four blocks' pixels are fetched before the current
-// four are encoded, so each wave keeps two loads in flight.
+// Alpha-only chunk (BC4/BC5), one block per lane: the wave covers 64
+// consecutive blocks per step and loads the next step's pixels before
+// encoding the current ones.
 template <int FMT>
-__device__ __forceinline__ void run_chunk_light(const BatchTex& T, uint32_t first, uint32_t count, int g, int l)
+__device__ __forceinline__ void run_chunk_light(const BatchTex& T, uint32_t first, uint32_t count, int lane)
 {
-    bool v_next;
-    uint32_t px_next = chunk_pixel(T, first, count, 0, g, l, v_next);
-    for (uint32_t off = 0; off < count; off += 4) {
-        const uint32_t b = first + off + (uint32_t)g;
-        const bool live = off + (uint32_t)g < count;
-        const bool valid = v_next;
-        const uint32_t px = px_next;
-        px_next = chunk_pixel(T, first, count, off + 4, g, l, v_next);
-        uint32_t w[4];
-        encode_alpha_only<FMT>(px, valid, l, w);
-        if (live && l == 0) {
+    auto fetch = [&](uint32_t off, uint32_t (&px)[16]) -> uint32_t {
+        const uint32_t k = off + (uint32_t)lane;
+        if (k >= count) return 0u;
+        const uint32_t b = first + k;
+        return fetch_block_lane(T.rgba, T.width, T.height, b % T.blocks_wide, b / T.blocks_wide, px);
+    };
+    uint32_t px[16], nx[16];
+    uint32_t valid = fetch(0, px);
+    for (uint32_t off = 0; off < count; off += 64) {
+        const uint32_t nvalid = fetch(off + 64, nx);
+        if (off + (uint32_t)lane < count) {
+            uint32_t w[4];
+            encode_alpha_lane<FMT>(px, valid, w);
+            const size_t b = (size_t)first + off + (uint32_t)lane;
             if (FMT == BC4)
-                *reinterpret_cast<uint2*>(T.out + (size_t)b * 8u) = make_uint2(w[0], w[1]);
+                *reinterpret_cast<uint2*>(T.out + b * 8u) = make_uint2(w[0], w[1]);
             else
-                *reinterpret_cast<uint4*>(T.out + (size_t)b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
+                *reinterpret_cast<uint4*>(T.out + b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
         }
+#pragma unroll
+        for (int p = 0; p < 16; ++p) px[p] = nx[p];
+        valid = nvalid;
     }
 }
 
@@ -118,7 +115,6 @@ __global__ __launch_bounds__(256) void k_batch_light(const BatchTex* __restrict_
                                                      uint32_t nchunks, uint32_t* __restrict__ counter)
 {
     const int lane = (int)threadIdx.x & 63;
-    const int g = lane >> 4, l = lane & 15;
     for (;;) {
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(counter, 1u);
@@ -126,8 +122,8 @@ __global__ __launch_bounds__(256) void k_batch_light(const BatchTex* __restrict_
         if (c >= nchunks) break;
         const uint4 ch = chunks[first + c];
         const BatchTex& T = tex[ch.x];
-        if (T.variant == 9u) run_chunk_light<BC4>(T, ch.y, ch.z, g, l);
-        else run_chunk_light<BC5>(T, ch.y, ch.z, g, l);
+        if (T.variant == 9u) run_chunk_light<BC4>(T, ch.y, ch.z, lane);
+        else run_chunk_light<BC5>(T, ch.y, ch.z, lane);
     }
 }
 
@@ -211,6 +207,9 @@ bool valid_texture(const tx_texture& t)
     if (t.height > 0 && (!t.d_rgba || !t.d_output)) return false;
     return true;
 }
+
+// Blocks per chunk of an alpha-only (BC4/BC5) texture: 8 steps of 64 lanes.
+constexpr uint32_t kLightChunkBlocks = 512;
 
 // Relative per-block cost used to order chunks (most expensive first).
 uint32_t cost_class(const tx_texture& t)
@@ -325,8 +324,10 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
         d.wg = s.params.weights[1];
         d.wb = s.params.weights[2];
         d.weigh_alpha = s.params.weigh_colour_by_alpha ? 1u : 0u;
-        for (uint32_t f = 0; f < d.nblocks; f += t.chunk_blocks) {
-            chunks.push_back(make_uint4((uint32_t)i, f, std::min(t.chunk_blocks, d.nblocks - f), 0u));
+        // alpha-only textures: one block per lane, so larger chunks
+        const uint32_t cb = cost_class(s) == 0 ? kLightChunkBlocks : t.chunk_blocks;
+        for (uint32_t f = 0; f < d.nblocks; f += cb) {
+            chunks.push_back(make_uint4((uint32_t)i, f, std::min(cb, d.nblocks - f), 0u));
             cls.push_back(cost_class(s));
         }
         nblocks += nb;
