@@ -256,6 +256,48 @@ const char* tx_error_string(int code);
  * no device is usable. */
 const char* tx_device_arch(void);
 
+/* ---- Mip chains: the step before the encoder.  No counterpart in texpresso
+ * itself (it encodes the level it is given; its CLI writes level 0 only,
+ * cli/src/main.rs:148-160).  Level k is max(1, w >> k) x max(1, h >> k); each
+ * texel of level k+1 is (a + b + c + d + 2) >> 2 per channel over the 2x2 box
+ * of level k, edge-clamped. */
+
+/* Levels of a full chain down to 1x1 (0 when a side is 0). */
+uint32_t tx_mip_levels(size_t width, size_t height);
+/* Size of level `level`. */
+void tx_mip_dims(size_t width, size_t height, uint32_t level, size_t* width_out, size_t* height_out);
+/* RGBA8 bytes of levels 1..levels-1 stored back to back. */
+size_t tx_mip_chain_rgba_bytes(size_t width, size_t height, uint32_t levels);
+/* Compressed bytes of levels 0..levels-1 stored back to back (the data
+ * layout of a DDS file with mip levels). */
+size_t tx_mip_chain_compressed_size(int format, size_t width, size_t height, uint32_t levels);
+/* Device: levels 1..levels-1 of d_rgba into d_chain (4-byte aligned,
+ * chain_len >= tx_mip_chain_rgba_bytes).  Enqueued on `stream` (NULL: the
+ * library stream). */
+int tx_generate_mips_device(const uint8_t* d_rgba, size_t width, size_t height, uint32_t levels, uint8_t* d_chain,
+                            size_t chain_len, void* stream);
+/* Device: generates the chain in d_chain, then compresses every level
+ * (level 0 = d_rgba) back to back into d_output, level 0 first; each level
+ * is encoded exactly like tx_compress_device would encode it alone. */
+int tx_compress_mips_device(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
+                            uint32_t levels, uint8_t* d_chain, size_t chain_len, uint8_t* d_output, size_t out_len,
+                            void* stream);
+/* Host buffers: the same, synchronous. */
+int tx_compress_mips(int format, const uint8_t* rgba, size_t rgba_len, size_t width, size_t height,
+                     const tx_params* params, uint32_t levels, uint8_t* output, size_t out_len);
+
+/* ---- DDS container: the step after the encoder, replacing the ddsfile calls
+ * of texpresso's CLI (write: cli/src/main.rs:143-164, DXGI mapping 220-228;
+ * read: 167-196, 230-248).  Header = "DDS " + DDS_HEADER + DDS_HEADER_DXT10
+ * (148 bytes): Texture2D, DXGI BC1-3 UNORM_SRGB / BC4-5 UNORM, alpha mode
+ * premultiplied for BC1 and straight otherwise; level data follows. */
+size_t tx_dds_header_size(void);
+int tx_dds_write_header(int format, size_t width, size_t height, uint32_t levels, uint8_t* out, size_t out_len);
+/* Accepts the five DXGI formats above with a Texture2D dimension, or a
+ * legacy DXT1/DXT3/DXT5 FourCC header; anything else is TX_ERR_FORMAT. */
+int tx_dds_parse_header(const uint8_t* data, size_t len, int* format, size_t* width, size_t* height,
+                        uint32_t* levels, size_t* data_offset);
+
 #ifdef __cplusplus
 }
 #endif

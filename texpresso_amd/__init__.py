@@ -15,7 +15,7 @@ import os
 
 __all__ = [
     "Format", "Algorithm", "Params", "TexpressoError", "COLOUR_WEIGHTS_UNIFORM", "COLOUR_WEIGHTS_PERCEPTUAL",
-    "num_blocks", "lib", "library_path",
+    "num_blocks", "lib", "library_path", "mip_levels", "mip_dims",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -189,6 +189,19 @@ def lib():
             "tx_server_fetch": (c_int, [c_vp, ctypes.c_uint64, c_vp, c_sz]),
             "tx_server_release": (c_int, [c_vp, ctypes.c_uint64]),
             "tx_server_destroy": (c_int, [c_vp]),
+            "tx_mip_levels": (ctypes.c_uint32, [c_sz, c_sz]),
+            "tx_mip_dims": (None, [c_sz, c_sz, ctypes.c_uint32, ctypes.POINTER(c_sz), ctypes.POINTER(c_sz)]),
+            "tx_mip_chain_rgba_bytes": (c_sz, [c_sz, c_sz, ctypes.c_uint32]),
+            "tx_mip_chain_compressed_size": (c_sz, [c_int, c_sz, c_sz, ctypes.c_uint32]),
+            "tx_generate_mips_device": (c_int, [c_vp, c_sz, c_sz, ctypes.c_uint32, c_vp, c_sz, c_vp]),
+            "tx_compress_mips_device": (c_int, [c_int, c_vp, c_sz, c_sz, pp, ctypes.c_uint32, c_vp, c_sz, c_vp, c_sz,
+                                                c_vp]),
+            "tx_compress_mips": (c_int, [c_int, c_u8p, c_sz, c_sz, c_sz, pp, ctypes.c_uint32, c_u8p, c_sz]),
+            "tx_dds_header_size": (c_sz, []),
+            "tx_dds_write_header": (c_int, [c_int, c_sz, c_sz, ctypes.c_uint32, c_u8p, c_sz]),
+            "tx_dds_parse_header": (c_int, [c_u8p, c_sz, ctypes.POINTER(c_int), ctypes.POINTER(c_sz),
+                                            ctypes.POINTER(c_sz), ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(c_sz)]),
             "tx_error_string": (ctypes.c_char_p, [c_int]),
             "tx_device_arch": (ctypes.c_char_p, []),
         }
@@ -291,6 +304,52 @@ class Format(enum.IntEnum):
 
     def decompress_device(self, d_data, width, height, d_output, stream=None):
         _check(lib().tx_decompress_device(int(self), d_data, width, height, d_output, stream), "decompress_device")
+
+    # -- mip chains (no counterpart in texpresso: it encodes one level) ------
+
+    def mip_chain_size(self, width, height, levels=None):
+        """Compressed bytes of levels 0..levels-1 back to back (default: full chain)."""
+        levels = mip_levels(width, height) if levels is None else levels
+        return lib().tx_mip_chain_compressed_size(int(self), width, height, levels)
+
+    def compress_mips(self, rgba, width, height, params, levels=None):
+        """Generate the mip chain of `rgba` on the GPU and compress every level;
+        returns the levels' blocks back to back (level 0 first) as a uint8 array."""
+        import numpy as np
+        levels = mip_levels(width, height) if levels is None else levels
+        out = np.zeros(max(1, self.mip_chain_size(width, height, levels)), np.uint8)
+        p = params._c()
+        ip, il, _k1 = _ptr(rgba)
+        _check(lib().tx_compress_mips(int(self), ip, il, width, height, ctypes.byref(p), levels, out.ctypes.data,
+                                      out.nbytes), "compress_mips")
+        return out[:self.mip_chain_size(width, height, levels)]
+
+    def compress_mips_device(self, d_rgba, width, height, params, levels, d_chain, chain_len, d_output, out_len,
+                             stream=None):
+        p = params._c()
+        _check(lib().tx_compress_mips_device(int(self), d_rgba, width, height, ctypes.byref(p), levels, d_chain,
+                                             chain_len, d_output, out_len, stream), "compress_mips_device")
+
+
+def mip_levels(width, height):
+    """Levels of a full mip chain down to 1x1."""
+    return lib().tx_mip_levels(width, height)
+
+
+def mip_dims(width, height, level):
+    w, h = ctypes.c_size_t(), ctypes.c_size_t()
+    lib().tx_mip_dims(width, height, level, ctypes.byref(w), ctypes.byref(h))
+    return w.value, h.value
+
+
+def mip_chain_rgba_bytes(width, height, levels):
+    return lib().tx_mip_chain_rgba_bytes(width, height, levels)
+
+
+def generate_mips_device(d_rgba, width, height, levels, d_chain, chain_len, stream=None):
+    """Levels 1..levels-1 of the device image d_rgba into d_chain."""
+    _check(lib().tx_generate_mips_device(d_rgba, width, height, levels, d_chain, chain_len, stream),
+           "generate_mips_device")
 
 
 def _is_nested(x):

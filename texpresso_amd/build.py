@@ -15,8 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtexpresso_hip.so")
-SOURCES = ["bcn_kernels.hip", "tx_batch.hip", "tx_capi.hip", "tx_context.hip", "tx_pipeline.hip", "tx_probe.hip",
-           "tx_server.hip"]
+SOURCES = ["bcn_kernels.hip", "tx_batch.hip", "tx_capi.hip", "tx_context.hip", "tx_dds.hip", "tx_mips.hip",
+           "tx_pipeline.hip", "tx_probe.hip", "tx_server.hip"]
 ARCH = "gfx950"
 
 

@@ -52,5 +52,8 @@ hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32
 hipError_t launch_encode(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s);
 hipError_t launch_decode(int format, const uint8_t* data, uint32_t w, uint32_t h, uint8_t* out, hipStream_t s);
+// One mip step of the RGBA8 image src (w x h, 4-byte aligned) into dst
+// (max(1, w/2) x max(1, h/2)): 2x2 box, edge clamp, (sum + 2) >> 2.
+hipError_t launch_mip_down(const uint8_t* src, uint32_t w, uint32_t h, uint8_t* dst, hipStream_t s);
 
 }  // namespace txk

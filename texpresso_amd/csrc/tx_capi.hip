@@ -219,6 +219,154 @@ int tx_compress_device(int format, const uint8_t* d_rgba, size_t width, size_t h
     return TX_OK;
 }
 
+// ------------------------------------------------------------ mip chains
+
+static size_t mip_side(size_t v, uint32_t level) { return level >= 64 ? 1 : std::max<size_t>(1, v >> level); }
+
+uint32_t tx_mip_levels(size_t width, size_t height)
+{
+    if (width == 0 || height == 0) return 0;
+    uint32_t n = 1;
+    for (size_t m = std::max(width, height); m > 1; m >>= 1) ++n;
+    return n;
+}
+
+void tx_mip_dims(size_t width, size_t height, uint32_t level, size_t* w_out, size_t* h_out)
+{
+    if (w_out) *w_out = mip_side(width, level);
+    if (h_out) *h_out = mip_side(height, level);
+}
+
+size_t tx_mip_chain_rgba_bytes(size_t width, size_t height, uint32_t levels)
+{
+    size_t total = 0;
+    for (uint32_t k = 1; k < levels; ++k) total += 4 * mip_side(width, k) * mip_side(height, k);
+    return total;
+}
+
+size_t tx_mip_chain_compressed_size(int format, size_t width, size_t height, uint32_t levels)
+{
+    size_t total = 0;
+    for (uint32_t k = 0; k < levels; ++k) total += tx_compressed_size(format, mip_side(width, k), mip_side(height, k));
+    return total;
+}
+
+// Shared checks of the mip-chain entry points.
+static int check_mips(size_t width, size_t height, uint32_t levels, size_t chain_len)
+{
+    if (width == 0) return TX_ERR_ZERO_WIDTH;
+    if (height == 0 || width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return TX_ERR_INVALID;
+    if (levels == 0 || levels > tx_mip_levels(width, height)) return TX_ERR_INVALID;
+    if (chain_len < tx_mip_chain_rgba_bytes(width, height, levels)) return TX_ERR_OUTPUT_SHORT;
+    return TX_OK;
+}
+
+// Levels 1..levels-1 into d_chain, each from the previous one.
+static hipError_t mips_on(const uint8_t* d_rgba, size_t width, size_t height, uint32_t levels, uint8_t* d_chain,
+                          hipStream_t s)
+{
+    const uint8_t* src = d_rgba;
+    size_t off = 0;
+    for (uint32_t k = 1; k < levels; ++k) {
+        uint8_t* dst = d_chain + off;
+        const hipError_t e = txk::launch_mip_down(src, (uint32_t)mip_side(width, k - 1),
+                                                  (uint32_t)mip_side(height, k - 1), dst, s);
+        if (e != hipSuccess) return e;
+        src = dst;
+        off += 4 * mip_side(width, k) * mip_side(height, k);
+    }
+    return hipSuccess;
+}
+
+// Every level of the chain (level 0 = d_rgba, then d_chain) compressed back
+// to back into d_output, level 0 first.
+static hipError_t compress_chain_on(txc::DeviceContext* ctx, int format, const tx_params* params,
+                                    const uint8_t* d_rgba, size_t width, size_t height, uint32_t levels,
+                                    const uint8_t* d_chain, uint8_t* d_output, hipStream_t s)
+{
+    const uint8_t* src = d_rgba;
+    size_t in_off = 0, out_off = 0;
+    for (uint32_t k = 0; k < levels; ++k) {
+        const size_t w = mip_side(width, k), h = mip_side(height, k);
+        const size_t n = tx_compressed_size(format, w, h);
+        if (k > 0) {
+            src = d_chain + in_off;
+            in_off += 4 * w * h;
+        }
+        const hipError_t e = encode_on(ctx, encode_args(format, params, src, w, h, d_output + out_off, n), s);
+        if (e != hipSuccess) return e;
+        out_off += n;
+    }
+    return hipSuccess;
+}
+
+int tx_generate_mips_device(const uint8_t* d_rgba, size_t width, size_t height, uint32_t levels, uint8_t* d_chain,
+                            size_t chain_len, void* stream)
+{
+    int rc = check_mips(width, height, levels, chain_len);
+    if (rc != TX_OK) return rc;
+    if (levels == 1) return TX_OK;
+    if (!d_rgba || !d_chain) return TX_ERR_INVALID;
+    if (((uintptr_t)d_rgba & 3u) || ((uintptr_t)d_chain & 3u)) return TX_ERR_ALIGN;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    return mips_on(d_rgba, width, height, levels, d_chain, stream ? (hipStream_t)stream : ctx->stream) == hipSuccess
+               ? TX_OK
+               : TX_ERR_HIP;
+}
+
+int tx_compress_mips_device(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
+                            uint32_t levels, uint8_t* d_chain, size_t chain_len, uint8_t* d_output, size_t out_len,
+                            void* stream)
+{
+    if (!valid_format(format) || !valid_params(params)) return TX_ERR_FORMAT;
+    int rc = check_mips(width, height, levels, chain_len);
+    if (rc != TX_OK) return rc;
+    if (out_len < tx_mip_chain_compressed_size(format, width, height, levels)) return TX_ERR_OUTPUT_SHORT;
+    if (!d_rgba || !d_output || (levels > 1 && !d_chain)) return TX_ERR_INVALID;
+    if (((uintptr_t)d_rgba & 3u) || ((uintptr_t)d_chain & 3u) || ((uintptr_t)d_output & (tx_block_size(format) - 1)))
+        return TX_ERR_ALIGN;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (mips_on(d_rgba, width, height, levels, d_chain, s) != hipSuccess) return TX_ERR_HIP;
+    if (compress_chain_on(ctx, format, params, d_rgba, width, height, levels, d_chain, d_output, s) != hipSuccess)
+        return TX_ERR_HIP;
+    return TX_OK;
+}
+
+int tx_compress_mips(int format, const uint8_t* rgba, size_t rgba_len, size_t width, size_t height,
+                     const tx_params* params, uint32_t levels, uint8_t* output, size_t out_len)
+{
+    if (!valid_format(format) || !valid_params(params)) return TX_ERR_FORMAT;
+    const size_t chain_len = tx_mip_chain_rgba_bytes(width, height, levels);
+    int rc = check_mips(width, height, levels, chain_len);
+    if (rc != TX_OK) return rc;
+    size_t in_bytes = 0;
+    if (!image_bytes(width, height, &in_bytes)) return TX_ERR_INVALID;
+    if (rgba_len < in_bytes) return TX_ERR_INPUT_SHORT;
+    const size_t need = tx_mip_chain_compressed_size(format, width, height, levels);
+    if (out_len < need) return TX_ERR_OUTPUT_SHORT;
+    if (!rgba || !output) return TX_ERR_INVALID;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    uint8_t* d = ctx->scratch(up(in_bytes) + up(chain_len) + need);
+    if (!d) return TX_ERR_NOMEM;
+    uint8_t* d_in = d;
+    uint8_t* d_chain = d + up(in_bytes);
+    uint8_t* d_out = d_chain + up(chain_len);
+    if (hipMemcpyAsync(d_in, rgba, in_bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    if (mips_on(d_in, width, height, levels, d_chain, ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    if (compress_chain_on(ctx, format, params, d_in, width, height, levels, d_chain, d_out, ctx->stream) !=
+        hipSuccess)
+        return TX_ERR_HIP;
+    if (hipMemcpyAsync(output, d_out, need, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return TX_ERR_HIP;
+    return TX_OK;
+}
+
 int tx_compress_block_masked(int format, const uint8_t rgba[64], uint32_t mask, const tx_params* params,
                              uint8_t* output, size_t out_len)
 {
