@@ -501,6 +501,17 @@ def main():
     }
     if phases is not None:
         line["secondary"]["pipeline_ms_per_step"] = {k: round(v / args.steps, 4) for k, v in phases.items()}
+    # host buffers in and out (tx_compress: PCIe upload, encode, download), wall clock
+    host_out = np.zeros(out_len, np.uint8)
+    flat = np.ascontiguousarray(img.reshape(-1))
+    fmt.compress(flat, W, H, params, host_out)
+    t_host = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        fmt.compress(flat, W, H, params, host_out)
+        t_host.append(time.perf_counter() - t0)
+    line["secondary"]["host_buffers_ms"] = round(1e3 * float(np.median(t_host)), 4)
+    line["secondary"]["host_buffers_mpix_s"] = round(mpix / float(np.median(t_host)), 1)
 
     import ctypes
     probe = ctypes.c_double()
