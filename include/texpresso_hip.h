@@ -169,7 +169,11 @@ typedef struct {
     uint32_t max_inflight;         /* concurrent launches (streams), 1..8 */
     uint32_t max_jobs_per_launch;  /* jobs folded into one launch */
     uint32_t max_pending;          /* submissions waiting for admission */
-    tx_batch_tuning tuning;        /* chunking and grid of each launch */
+    tx_batch_tuning tuning;        /* chunking and grid of each launch; with tuning.pipeline = 1
+                                      (default) launches of >= tx_set_pipeline_threshold BC1-BC3
+                                      blocks (and <= 4M) use the binned pipeline, whose records
+                                      (336 B per block for min(4M, hbm_budget/64) blocks per
+                                      in-flight launch) are allocated when the server is created */
 } tx_server_config;
 
 typedef struct {

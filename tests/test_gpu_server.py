@@ -116,3 +116,23 @@ def test_wait_timeout_and_pending_bound(gpu):
         assert srv.wait(jid).status == 3
         srv.release(jid)
     srv.close()
+
+
+@pytest.mark.parametrize("path", ["pipeline", "lockstep"])
+def test_launch_paths_agree_with_the_oracle(gpu, oracle, path):
+    # the same mixed jobs with every launch forced through the binned pipeline,
+    # then through the lockstep kernel (alpha-only jobs use their own kernel in both)
+    old = tx.set_pipeline_threshold(0 if path == "pipeline" else 2 ** 64 - 1)
+    try:
+        jobs = make_jobs(7, 24, lo=4, hi=300)
+        srv = tx.Server(hbm_budget=512 << 20, max_inflight=2, max_jobs_per_launch=8)
+        alone = run_alone(srv, jobs[:6])
+        mixed, infos = run_mixed(srv, jobs)
+        srv.close()
+    finally:
+        tx.set_pipeline_threshold(old)
+    for k, (fmt, p, img, w, h) in enumerate(jobs):
+        want = oracle.compress(fmt, img, w, h, oracle.params(int(p.algorithm), p.weights, False))
+        assert np.array_equal(mixed[k], want), "job %d (%s) differs from the oracle" % (k, path)
+        if k < 6:
+            assert np.array_equal(alone[k], want), "job %d alone (%s)" % (k, path)

@@ -49,6 +49,18 @@ hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32
                             uint32_t* d_counter, uint32_t grid, uint32_t grid_light, uint32_t waves_per_group,
                             hipStream_t s, hipStream_t s_light, hipEvent_t fork, hipEvent_t join);
 
+// Blocks per chunk of an alpha-only (BC4/BC5) texture in the persistent
+// kernels: the alpha-only kernel runs one block per lane, 8 steps of 64.
+constexpr uint32_t kLightChunkBlocks = 512;
+
+// Counter reset (d_counter[0..1]) + the alpha-only persistent kernel over
+// chunks [first, first + nlight) on `s`.
+hipError_t launch_batch_light(const BatchTex* d_tex, const uint4* d_chunks, uint32_t first, uint32_t nlight,
+                              uint32_t* d_counter, uint32_t grid_light, hipStream_t s);
+
+// Clears d_counter[0..1] on `s` (a kernel: replay-safe inside graphs).
+hipError_t reset_counters(uint32_t* d_counter, hipStream_t s);
+
 hipError_t launch_encode(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s);
 hipError_t launch_decode(int format, const uint8_t* data, uint32_t w, uint32_t h, uint8_t* out, hipStream_t s);

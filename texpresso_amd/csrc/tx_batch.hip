@@ -139,6 +139,22 @@ __global__ void k_reset_counter(uint32_t* counter)
 
 namespace txk {
 
+hipError_t reset_counters(uint32_t* d_counter, hipStream_t s)
+{
+    hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_light(const BatchTex* d_tex, const uint4* d_chunks, uint32_t first, uint32_t nlight,
+                              uint32_t* d_counter, uint32_t grid_light, hipStream_t s)
+{
+    if (nlight == 0) return hipSuccess;
+    hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
+    hipLaunchKernelGGL(txd::k_batch_light, dim3(grid_light), dim3(256), 0, s, d_tex, d_chunks, first, nlight,
+                       d_counter);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nheavy, uint32_t nlight,
                             uint32_t* d_counter, uint32_t grid, uint32_t grid_light, uint32_t waves_per_group,
                             hipStream_t s, hipStream_t s_light, hipEvent_t fork, hipEvent_t join)
@@ -207,9 +223,6 @@ bool valid_texture(const tx_texture& t)
     if (t.height > 0 && (!t.d_rgba || !t.d_output)) return false;
     return true;
 }
-
-// Blocks per chunk of an alpha-only (BC4/BC5) texture: 8 steps of 64 lanes.
-constexpr uint32_t kLightChunkBlocks = 512;
 
 // Relative per-block cost used to order chunks (most expensive first).
 uint32_t cost_class(const tx_texture& t)
@@ -325,7 +338,7 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
         d.wb = s.params.weights[2];
         d.weigh_alpha = s.params.weigh_colour_by_alpha ? 1u : 0u;
         // alpha-only textures: one block per lane, so larger chunks
-        const uint32_t cb = cost_class(s) == 0 ? kLightChunkBlocks : t.chunk_blocks;
+        const uint32_t cb = cost_class(s) == 0 ? txk::kLightChunkBlocks : t.chunk_blocks;
         for (uint32_t f = 0; f < d.nblocks; f += cb) {
             chunks.push_back(make_uint4((uint32_t)i, f, std::min(cb, d.nblocks - f), 0u));
             cls.push_back(cost_class(s));

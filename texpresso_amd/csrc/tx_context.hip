@@ -32,6 +32,8 @@ uint8_t* DeviceContext::scratch(size_t bytes)
         pool = nullptr;
         return nullptr;
     }
+    // first access to fresh device memory is slow; pay it here, once
+    (void)hipMemsetAsync(pool, 0, cap, stream);
     pool_cap = cap;
     return pool;
 }
@@ -48,6 +50,14 @@ bool DeviceContext::pipe_scratch(uint64_t recs, txp::PipeBufs* out)
             pipe_recs = 0;
         }
         if (hipMalloc(&pipe, txp::PipeBufs::bytes_for(cap)) != hipSuccess) {
+            pipe = nullptr;
+            return false;
+        }
+        // touch it once (first access to fresh device memory is slow); the
+        // scratch may next be used on another stream, so wait here
+        if (hipMemsetAsync(pipe, 0, txp::PipeBufs::bytes_for(cap), stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            (void)hipFree(pipe);
             pipe = nullptr;
             return false;
         }

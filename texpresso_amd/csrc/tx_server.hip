@@ -107,11 +107,18 @@ struct Launch {
     uint4* h_chunks = nullptr;       // pinned
     txk::BatchTex* d_tex = nullptr;
     uint4* d_chunks = nullptr;
-    uint32_t* d_counter = nullptr;
+    uint32_t* d_counter = nullptr;   // [0..1] colour kernel, [2..3] alpha-only kernel
     size_t chunk_cap = 0;
+    uint8_t* pipe_mem = nullptr;     // binned-pipeline records of this slot (grown on demand)
+    txp::PipeBufs pipe;
     std::vector<uint64_t> jobs;
     bool busy = false;
 };
+
+// Largest number of BC1-BC3 blocks a launch sends through the binned
+// pipeline (records: 336 bytes each, so 1.4 GB per launch slot at most);
+// larger launches use the lockstep persistent kernel.
+constexpr uint64_t kServerPipeMaxBlocks = 4u << 20;
 
 }  // namespace
 
@@ -119,6 +126,7 @@ struct tx_server {
     tx_server_config cfg{};
     int device = -1;
     uint32_t grid = 0;
+    uint32_t cus = 0;
     Arena arena;
     std::vector<Launch> launches;
     std::mutex mu;
@@ -165,6 +173,23 @@ uint32_t variant_of(const tx_job& s)
     return s.format >= TX_BC4 ? 9u + (uint32_t)(s.format - TX_BC4) : 3u * (uint32_t)s.format + (uint32_t)s.params.algorithm;
 }
 
+// Record capacity of a slot's pipeline scratch: at least `blocks`.  The slot
+// is idle (its previous launch has retired), so the old buffer can go.
+bool grow_pipe(Launch& L, uint64_t blocks)
+{
+    if (L.pipe_mem && L.pipe.capacity >= blocks) return true;
+    const uint64_t cap = std::min<uint64_t>(kServerPipeMaxBlocks, std::max<uint64_t>(blocks, 2 * L.pipe.capacity));
+    if (L.pipe_mem) (void)hipFree(L.pipe_mem);
+    L.pipe_mem = nullptr;
+    L.pipe = txp::PipeBufs{};
+    if (hipMalloc(&L.pipe_mem, txp::PipeBufs::bytes_for(cap)) != hipSuccess) {
+        L.pipe_mem = nullptr;
+        return false;
+    }
+    L.pipe.carve(L.pipe_mem, cap);
+    return true;
+}
+
 }  // namespace
 
 // Called with the lock held; drops it while enqueueing device work.
@@ -172,22 +197,28 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
 {
     // chunks small enough that every resident wave gets work even when the
     // launch holds a single small job (latency), capped by the tuned size
-    uint64_t total_blocks = 0;
+    uint64_t colour_blocks = 0;  // BC1-BC3 (LDS encoder or pipeline); BC4/BC5 go to the alpha-only kernel
+    bool any_bc1_cluster = false;
     for (uint64_t id : ids) {
         const Job& j = jobs[id];
-        total_blocks += (uint64_t)((j.spec.width + 3) / 4) * ((j.spec.height + 3) / 4);
+        if (j.spec.format <= TX_BC3)
+            colour_blocks += (uint64_t)((j.spec.width + 3) / 4) * ((j.spec.height + 3) / 4);
+        any_bc1_cluster |= j.spec.format == TX_BC1 && j.spec.params.algorithm != TX_RANGE_FIT;
     }
     const uint64_t waves = (uint64_t)grid * cfg.tuning.waves_per_group;
-    uint64_t want = (total_blocks + waves - 1) / std::max<uint64_t>(1, waves);
+    uint64_t want = (colour_blocks + waves - 1) / std::max<uint64_t>(1, waves);
     want = std::max<uint64_t>(4, (want + 3) & ~(uint64_t)3);
     const uint32_t chunk = (uint32_t)std::min<uint64_t>(cfg.tuning.chunk_blocks, want);
+    const bool pipeline = cfg.tuning.pipeline && colour_blocks >= txc::pipeline_min_blocks() &&
+                          colour_blocks <= kServerPipeMaxBlocks;
     size_t nchunks = 0;
     std::vector<tx_job> specs;
     std::vector<size_t> in_offs, out_offs;
     for (uint64_t id : ids) {
         Job& j = jobs[id];
         const uint64_t nb = (uint64_t)((j.spec.width + 3) / 4) * ((j.spec.height + 3) / 4);
-        nchunks += (size_t)((nb + chunk - 1) / chunk);
+        const uint32_t cb = j.spec.format >= TX_BC4 ? txk::kLightChunkBlocks : chunk;
+        nchunks += (size_t)((nb + cb - 1) / cb);
         specs.push_back(j.spec);
         in_offs.push_back(j.in_off);
         out_offs.push_back(j.out_off);
@@ -204,6 +235,8 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
     lock.unlock();
 
     bool ok = grow_chunks(L, std::max<size_t>(1, nchunks));
+    // without room for the records the launch falls back to the lockstep kernel
+    const bool use_pipe = ok && pipeline && grow_pipe(L, colour_blocks);
     uint8_t* base = arena.base();
     size_t c = 0;
     // expensive jobs first so the cheap ones fill the tail
@@ -231,12 +264,24 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
         t.wb = s.params.weights[2];
         t.weigh_alpha = s.params.weigh_colour_by_alpha ? 1u : 0u;
     }
+    // colour chunks first (each with its first record slot for the pipeline),
+    // then the alpha-only chunks
+    uint32_t nheavy = 0, slot = 0;
     for (size_t q = 0; ok && q < order.size(); ++q) {
         const size_t k = order[q];
         const txk::BatchTex& t = L.h_tex[k];
-        for (uint32_t f = 0; f < t.nblocks; f += chunk)
-            L.h_chunks[c++] = make_uint4((uint32_t)k, f, std::min(chunk, t.nblocks - f), 0u);
+        const bool light = t.variant >= 9u;
+        const uint32_t cb = light ? txk::kLightChunkBlocks : chunk;
+        for (uint32_t f = 0; f < t.nblocks; f += cb) {
+            const uint32_t n = std::min(cb, t.nblocks - f);
+            L.h_chunks[c++] = make_uint4((uint32_t)k, f, n, light ? 0u : slot);
+            if (!light) {
+                slot += n;
+                ++nheavy;
+            }
+        }
     }
+    const uint32_t nlight = (uint32_t)c - nheavy;
     if (ok && hipMemcpyAsync(L.d_tex, L.h_tex, sizeof(txk::BatchTex) * specs.size(), hipMemcpyHostToDevice,
                              L.stream) != hipSuccess)
         ok = false;
@@ -244,12 +289,28 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
                        hipSuccess)
         ok = false;
     // no more persistent workgroups than chunks
-    const uint32_t launch_grid =
-        (uint32_t)std::min<uint64_t>(grid, (c + cfg.tuning.waves_per_group - 1) / cfg.tuning.waves_per_group);
-    if (ok && c &&
-        txk::launch_batch_raw(L.d_tex, L.d_chunks, (uint32_t)c, 0u, L.d_counter, launch_grid, 0u,
-                              cfg.tuning.waves_per_group, L.stream, nullptr, nullptr, nullptr) != hipSuccess)
+    const uint32_t launch_grid = (uint32_t)std::min<uint64_t>(
+        grid, std::max<uint64_t>(1, (nheavy + cfg.tuning.waves_per_group - 1) / cfg.tuning.waves_per_group));
+    const uint32_t light_grid = (uint32_t)std::min<uint64_t>((uint64_t)cus * 8, std::max<uint32_t>(1, (nlight + 3) / 4));
+    if (ok && use_pipe) {
+        if (txk::launch_batch_light(L.d_tex, L.d_chunks, nheavy, nlight, L.d_counter + 2, light_grid, L.stream) !=
+            hipSuccess)
+            ok = false;
+        txp::PipeLaunch p{};
+        p.d_tex = L.d_tex;
+        p.d_chunks = L.d_chunks;
+        p.nchunks = nheavy;
+        p.nslots = slot;
+        p.chunk_blocks = chunk;
+        p.classify_grid = std::min<uint32_t>(cus * txp::kClassifyWavesPerCu, nheavy);
+        p.search_grid = (uint32_t)std::min<uint64_t>((uint64_t)cus * txp::kSearchWavesPerCu, (slot + 3) / 4);
+        p.any_bc1 = any_bc1_cluster;
+        if (ok && txp::launch_pipeline(p, L.pipe, L.stream) != hipSuccess) ok = false;
+    } else if (ok && c &&
+               txk::launch_batch_raw(L.d_tex, L.d_chunks, nheavy, nlight, L.d_counter, launch_grid, light_grid,
+                                     cfg.tuning.waves_per_group, L.stream, nullptr, nullptr, nullptr) != hipSuccess) {
         ok = false;
+    }
     if (ok && hipEventRecord(L.done, L.stream) != hipSuccess) ok = false;
     lock.lock();
     if (!ok) {
@@ -345,7 +406,7 @@ tx_server_config tx_server_config_default(void)
     c.max_pending = 4096;
     c.tuning = tx_batch_tuning_default();
     c.tuning.use_graph = 0;
-    c.tuning.pipeline = 0;  // launches use the single persistent kernel
+    c.tuning.pipeline = 1;  // launches with many ClusterFit blocks use the binned pipeline
     return c;
 }
 
@@ -359,7 +420,7 @@ int tx_server_create(tx_server** out, const tx_server_config* config)
         return TX_ERR_INVALID;
     if (cfg.tuning.chunk_blocks == 0 || cfg.tuning.chunk_blocks % 4 ||
         (cfg.tuning.waves_per_group != 1 && cfg.tuning.waves_per_group != 2 && cfg.tuning.waves_per_group != 4) ||
-        cfg.tuning.groups_per_cu == 0 || cfg.tuning.groups_per_cu > 64)
+        cfg.tuning.groups_per_cu == 0 || cfg.tuning.groups_per_cu > 64 || cfg.tuning.pipeline > 1)
         return TX_ERR_INVALID;
     int rc = TX_OK;
     txc::DeviceContext* ctx = txc::current_context(&rc);
@@ -371,6 +432,7 @@ int tx_server_create(tx_server** out, const tx_server_config* config)
     hipDeviceProp_t prop;
     bool ok = hipGetDeviceProperties(&prop, ctx->device) == hipSuccess;
     s->grid = ok ? (uint32_t)prop.multiProcessorCount * cfg.tuning.groups_per_cu : 0;
+    s->cus = ok ? (uint32_t)prop.multiProcessorCount : 0;
     ok = ok && s->arena.init(cfg.hbm_budget);
     ok = ok && hipStreamCreateWithFlags(&s->fetch_stream, hipStreamNonBlocking) == hipSuccess;
     s->launches.resize(cfg.max_inflight);
@@ -381,7 +443,26 @@ int tx_server_create(tx_server** out, const tx_server_config* config)
         ok = ok && hipMalloc(&L.d_tex, sizeof(txk::BatchTex) * cfg.max_jobs_per_launch) == hipSuccess;
         ok = ok && hipMalloc(&L.d_counter, 256) == hipSuccess;
         ok = ok && grow_chunks(L, 4096);
+        // record scratch for the largest launch the arena can hold (its
+        // inputs take 64 bytes per block), allocated once: a later free would
+        // wait for the whole device in the middle of serving
+        if (cfg.tuning.pipeline)
+            ok = ok && grow_pipe(L, std::min<uint64_t>(kServerPipeMaxBlocks, std::max<uint64_t>(1, cfg.hbm_budget / 64)));
     }
+    // touch the arena and the record scratch once now: the first access to
+    // fresh device memory costs milliseconds, which would otherwise land on
+    // the first jobs of each launch slot
+    ok = ok && hipMemsetAsync(s->arena.base(), 0, s->arena.size(), s->fetch_stream) == hipSuccess;
+    for (Launch& L : s->launches)
+        if (ok && L.pipe_mem)
+            ok = hipMemsetAsync(L.pipe_mem, 0, txp::PipeBufs::bytes_for(L.pipe.capacity), s->fetch_stream) ==
+                 hipSuccess;
+    ok = ok && hipStreamSynchronize(s->fetch_stream) == hipSuccess;
+    // a stream gets its hardware queue at its first dispatch: do that now
+    // (a counter reset) rather than in the first job of each slot
+    for (Launch& L : s->launches)
+        if (ok) ok = txk::reset_counters(L.d_counter, L.stream) == hipSuccess &&
+                     hipStreamSynchronize(L.stream) == hipSuccess;
     if (!ok) {
         tx_server_destroy(s);
         return TX_ERR_NOMEM;
@@ -511,6 +592,7 @@ int tx_server_destroy(tx_server* s)
         if (L.d_tex) (void)hipFree(L.d_tex);
         if (L.d_chunks) (void)hipFree(L.d_chunks);
         if (L.d_counter) (void)hipFree(L.d_counter);
+        if (L.pipe_mem) (void)hipFree(L.pipe_mem);
     }
     if (s->fetch_stream) (void)hipStreamDestroy(s->fetch_stream);
     s->arena.release();
