@@ -242,11 +242,13 @@ int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_
 int tx_util_time_phases(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
                         uint8_t* d_output, size_t out_len, int iters, float ms_phase[4], int* used_pipeline);
 
-/* ClusterFit textures with at least `min_blocks` blocks are encoded by the
- * binned pipeline (classify, counting sort by colour count, per-block
- * independent searches); smaller ones by the single direct kernel.  Both
- * give identical bytes.  0 = always the pipeline, UINT64_MAX = never.
- * Process-wide; returns the previous value (default 131072). */
+/* BC2/BC3 IterativeClusterFit textures with at least `min_blocks` blocks are
+ * encoded by the binned pipeline (classify, counting sort by colour count,
+ * per-block independent searches), smaller ones by the single direct kernel;
+ * BC1 IterativeClusterFit uses 1.5x, BC2/BC3 ClusterFit 2x and BC1
+ * ClusterFit 4x this threshold (the measured crossovers).  Both paths give identical bytes.  0 = always the
+ * pipeline, UINT64_MAX = never.  Process-wide; returns the previous value
+ * (default 131072). */
 uint64_t tx_set_pipeline_threshold(uint64_t min_blocks);
 
 /* Measures the device's plain FP32 VALU rate (separate multiply and add, no

@@ -125,8 +125,21 @@ static txk::EncodeArgs encode_args(int format, const tx_params* p, const uint8_t
     return a;
 }
 
+// Blocks from which a texture goes through the binned pipeline: the set
+// threshold for BC2/BC3 IterativeClusterFit, 1.5x for BC1
+// IterativeClusterFit, 2x for BC2/BC3 ClusterFit and 4x for BC1 ClusterFit
+// (measured crossovers: BC1's two search passes and single-ordering blocks
+// gain less from independent per-block scheduling).
+static uint64_t pipeline_threshold(int format, int algorithm)
+{
+    const uint64_t t = txc::pipeline_min_blocks();
+    const bool bc1 = format == TX_BC1;
+    const uint64_t halves = algorithm == TX_ITERATIVE_CLUSTER_FIT ? (bc1 ? 3 : 2) : (bc1 ? 8 : 4);
+    return t > UINT64_MAX / halves ? UINT64_MAX : t * halves / 2;
+}
+
 // Encodes one texture on `s`: ClusterFit textures of at least
-// pipeline_min_blocks() blocks go through the binned pipeline (classify,
+// pipeline_threshold() blocks go through the binned pipeline (classify,
 // bin, independent per-block searches), everything else through the direct
 // kernel.  Both produce the same bytes.  With `marks` (5 events) the
 // pipeline records them between its kernels; `used_pipeline` says which
@@ -136,7 +149,7 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
 {
     const bool cluster = a.format <= TX_BC3 && a.algorithm != TX_RANGE_FIT;
     if (used_pipeline) *used_pipeline = false;
-    if (!cluster || a.nblocks < txc::pipeline_min_blocks() || a.nblocks > 0xFFFFFFFFull)
+    if (!cluster || a.nblocks < pipeline_threshold(a.format, a.algorithm) || a.nblocks > 0xFFFFFFFFull)
         return txk::launch_encode(a, s);
     if (used_pipeline) *used_pipeline = true;
     std::lock_guard<std::mutex> lock(ctx->pipe_mu);

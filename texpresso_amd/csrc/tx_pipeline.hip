@@ -300,8 +300,9 @@ __global__ __launch_bounds__(64) void k_pipe_search(const BatchTex* __restrict__
     float axx = 0.0f, axy = 0.0f, axz = 0.0f;
     FitParams fp{0.0f, 0.0f, 0.0f, 0u};
     // the wave's private run of list positions [pool, pool_end), refilled
-    // kPull at a time with one global atomic
-    constexpr uint32_t kPull = 16;
+    // kPull at a time with one global atomic; kPull is at most the records
+    // per wave, so a short list still spreads over every wave
+    const uint32_t kPull = min(16u, max(4u, total / gridDim.x));
     uint32_t pool = 0, pool_end = 0;
     for (;;) {
         // ---- ordering phase
@@ -315,12 +316,14 @@ __global__ __launch_bounds__(64) void k_pipe_search(const BatchTex* __restrict__
                 const uint32_t avail = pool_end - pool;
                 uint32_t p = pool + rank;
                 if (avail < nwant) {
+                    // reserve enough for every asking group (nwant <= 4)
+                    const uint32_t grab = max(kPull, nwant);
                     uint32_t fresh = 0;
-                    if (lane == 0) fresh = atomicAdd(cursor, kPull);
+                    if (lane == 0) fresh = atomicAdd(cursor, grab);
                     fresh = __builtin_amdgcn_readfirstlane(fresh);
                     if (rank >= avail) p = fresh + (rank - avail);
                     pool = fresh + (nwant - avail);
-                    pool_end = fresh + kPull;
+                    pool_end = fresh + grab;
                 } else {
                     pool += nwant;
                 }
