@@ -626,18 +626,20 @@ __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t m
     // carried value back in the trip that consumed it keeps it in the same
     // registers (no copies at the back edge).  The kept minimum is the
     // smallest (error, code), so evaluation order does not matter.
+    // the lists are padded past the longest trip: reads need no guard, and
+    // entries at or past `mylen` are loaded but never evaluated
     const uint32_t t0 = (uint32_t)l;
-    uint32_t cA = t0 < mylen ? tab[t0] : 0u;
-    uint32_t cB = t0 + 16 < mylen ? tab[t0 + 16] : 0u;
+    uint32_t cA = tab[t0];
+    uint32_t cB = tab[t0 + 16];
     SplitSums sA = load_split<NC>(L, cA);
     txt::UnitSplit uA{};
-    if (UNIT && t0 < mylen) uA = ut[t0];
+    if (UNIT) uA = ut[t0];
     for (uint32_t base = 0; wave_any(base < mylen); base += 32) {
         const uint32_t t = base + (uint32_t)l;
-        const uint32_t cA2 = t + 32 < mylen ? tab[t + 32] : 0u;
+        const uint32_t cA2 = tab[t + 32];
         const SplitSums sB = load_split<NC>(L, cB);
         txt::UnitSplit uB{};
-        if (UNIT && t + 16 < mylen) uB = ut[t + 16];
+        if (UNIT) uB = ut[t + 16];
         if (t < mylen) {
             float a[3], b[3];
             const float err = eval_split<NC, UNIT>(sA, xs, fp, a, b, uA);
@@ -646,9 +648,9 @@ __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t m
                 bc = cA;
             }
         }
-        const uint32_t cB2 = t + 48 < mylen ? tab[t + 48] : 0u;
+        const uint32_t cB2 = tab[t + 48];
         sA = load_split<NC>(L, cA2);
-        if (UNIT && t + 32 < mylen) uA = ut[t + 32];
+        if (UNIT) uA = ut[t + 32];
         if (t + 16 < mylen) {
             float a[3], b[3];
             const float err = eval_split<NC, UNIT>(sB, xs, fp, a, b, uB);

@@ -36,14 +36,19 @@ constexpr int tri(int a, int b) { return ((a * (33 - a)) >> 1) + b; }
 constexpr int kTriEntries = 153;  // tri(16, 16) + 1
 constexpr int kCand4 = 967;       // compress4 splits for 16 colours
 constexpr int kCand3 = 151;       // compress3 splits for 16 colours
+// Stored lengths: the candidate loop reads up to 48 entries past its last
+// 32-candidate trip, so the lists are padded (code 0 = prefix entries S[0],
+// unit entries zero) and every read is in bounds without a guard.
+constexpr int kCand4Stored = 1024;
+constexpr int kCand3Stored = 192;
 
 struct SingleLut {
     uint8_t v[4][256][6];  // [5_3, 6_3, 5_4, 6_4][value][start0,end0,err0,start1,end1,err1]
 };
 
 struct CandTables {
-    uint32_t t4[kCand4];
-    uint32_t t3[kCand3];
+    uint32_t t4[kCand4Stored];
+    uint32_t t3[kCand3Stored];
     uint16_t len4[17];
     uint16_t len3[17];
 };
@@ -169,8 +174,8 @@ struct UnitSplit {
 };
 
 struct UnitTables {
-    UnitSplit u4[kCand4];
-    UnitSplit u3[kCand3];
+    UnitSplit u4[kCand4Stored];
+    UnitSplit u3[kCand3Stored];
 };
 
 constexpr UnitSplit unit_split4(int i, int j, int k)
