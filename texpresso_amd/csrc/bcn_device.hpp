@@ -65,6 +65,9 @@ static __constant__ txt::SingleLut c_lut = txt::make_single_lut();
 static __constant__ txt::CandTables c_cand = txt::make_cand_tables();
 static __constant__ txt::UnitTables c_unit = txt::make_unit_tables();
 
+// Word 3 of a raw (untyped) buffer resource on CDNA: 32-bit data format.
+constexpr int kBufferFlags = 0x00020000;
+
 // ---------------------------------------------------------------- helpers
 
 template <int N>
@@ -617,8 +620,17 @@ template <int NC, bool UNIT>
 __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t mylen, const float4& xs,
                                             const FitParams& fp, float& be, uint32_t& bc)
 {
-    const uint32_t* tab = NC == 4 ? c_cand.t4 : c_cand.t3;
-    const txt::UnitSplit* ut = NC == 4 ? c_unit.u4 : c_unit.u3;
+    // The code list is read through a buffer resource: one byte offset,
+    // advanced once per trip, with the +16/+32/+48 entries as immediate
+    // offsets (and hardware bounds checks: a read past the end returns 0).
+    // The 16-byte unit entries are read from a uniform base plus a 32-bit
+    // byte offset.
+    constexpr int kStored = NC == 4 ? txt::kCand4Stored : txt::kCand3Stored;
+    const __amdgpu_buffer_rsrc_t rt =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(NC == 4 ? c_cand.t4 : c_cand.t3), 0, kStored * 4, kBufferFlags);
+    const char* ubase = reinterpret_cast<const char*>(NC == 4 ? c_unit.u4 : c_unit.u3);
+    auto code_at = [&](uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b32(rt, off, 0, 0); };
+    auto unit_at = [&](uint32_t off) { return *reinterpret_cast<const txt::UnitSplit*>(ubase + off); };
     be = __builtin_huge_valf();
     bc = 0xFFFFFFFFu;
     // Two candidates per lane and trip (A at t, B at t + 16).  Codes are read
@@ -628,18 +640,18 @@ __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t m
     // smallest (error, code), so evaluation order does not matter.
     // the lists are padded past the longest trip: reads need no guard, and
     // entries at or past `mylen` are loaded but never evaluated
-    const uint32_t t0 = (uint32_t)l;
-    uint32_t cA = tab[t0];
-    uint32_t cB = tab[t0 + 16];
+    uint32_t ot = 4u * (uint32_t)l, ou = 16u * (uint32_t)l;  // byte offsets of entry t
+    uint32_t cA = code_at(ot);
+    uint32_t cB = code_at(ot + 64);
     SplitSums sA = load_split<NC>(L, cA);
     txt::UnitSplit uA{};
-    if (UNIT) uA = ut[t0];
+    if (UNIT) uA = unit_at(ou);
     for (uint32_t base = 0; wave_any(base < mylen); base += 32) {
         const uint32_t t = base + (uint32_t)l;
-        const uint32_t cA2 = tab[t + 32];
+        const uint32_t cA2 = code_at(ot + 128);
         const SplitSums sB = load_split<NC>(L, cB);
         txt::UnitSplit uB{};
-        if (UNIT) uB = ut[t + 16];
+        if (UNIT) uB = unit_at(ou + 256);
         if (t < mylen) {
             float a[3], b[3];
             const float err = eval_split<NC, UNIT>(sA, xs, fp, a, b, uA);
@@ -648,9 +660,9 @@ __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t m
                 bc = cA;
             }
         }
-        const uint32_t cB2 = tab[t + 48];
+        const uint32_t cB2 = code_at(ot + 192);
         sA = load_split<NC>(L, cA2);
-        if (UNIT) uA = ut[t + 32];
+        if (UNIT) uA = unit_at(ou + 512);
         if (t + 16 < mylen) {
             float a[3], b[3];
             const float err = eval_split<NC, UNIT>(sB, xs, fp, a, b, uB);
@@ -661,6 +673,8 @@ __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t m
         }
         cA = cA2;
         cB = cB2;
+        ot += 128;
+        ou += 512;
     }
 }
 
