@@ -110,3 +110,17 @@ def test_chain_sizes(w, h):
     for fmt in range(5):
         f = tx.Format(fmt)
         assert f.mip_chain_size(w, h) == sum(f.compressed_size(a, b) for a, b in dims)
+
+
+def test_mip_entry_points_reject_bad_sizes():
+    # argument checks run before any device work, so these need no GPU
+    import ctypes
+    L = tx.lib()
+    p = tx.Params()._c()
+    big = 0x7FFFFFFF
+    assert L.tx_compress_mips(0, None, 0, big, big, ctypes.byref(p), 1, None, 0) == -10  # 4*w*h overflows
+    assert L.tx_compress_mips(0, None, 0, 0, 4, ctypes.byref(p), 1, None, 0) == -2       # zero width
+    assert L.tx_compress_mips(0, None, 0, 4, 4, ctypes.byref(p), 4, None, 0) == -10      # 4x4 has 3 levels
+    assert L.tx_compress_mips(9, None, 0, 4, 4, ctypes.byref(p), 1, None, 0) == -1       # bad format
+    assert L.tx_mip_chain_rgba_bytes(2 ** 40, 2 ** 40, 3) == 2 ** 64 - 1                  # saturates
+    assert L.tx_mip_levels(0, 5) == 0 and L.tx_mip_levels(1, 1) == 1 and L.tx_mip_levels(4096, 3) == 13

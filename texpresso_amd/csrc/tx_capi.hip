@@ -253,7 +253,11 @@ void tx_mip_dims(size_t width, size_t height, uint32_t level, size_t* w_out, siz
 size_t tx_mip_chain_rgba_bytes(size_t width, size_t height, uint32_t levels)
 {
     size_t total = 0;
-    for (uint32_t k = 1; k < levels; ++k) total += 4 * mip_side(width, k) * mip_side(height, k);
+    for (uint32_t k = 1; k < levels; ++k) {
+        size_t b = 0;
+        if (!image_bytes(mip_side(width, k), mip_side(height, k), &b) || total > SIZE_MAX - b) return SIZE_MAX;
+        total += b;
+    }
     return total;
 }
 
@@ -269,6 +273,10 @@ static int check_mips(size_t width, size_t height, uint32_t levels, size_t chain
 {
     if (width == 0) return TX_ERR_ZERO_WIDTH;
     if (height == 0 || width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return TX_ERR_INVALID;
+    // the whole chain holds less than 4/3 of level 0: keep level 0 below half
+    // the address space so every size computed from it fits
+    size_t level0 = 0;
+    if (!image_bytes(width, height, &level0) || level0 > SIZE_MAX / 2) return TX_ERR_INVALID;
     if (levels == 0 || levels > tx_mip_levels(width, height)) return TX_ERR_INVALID;
     if (chain_len < tx_mip_chain_rgba_bytes(width, height, levels)) return TX_ERR_OUTPUT_SHORT;
     return TX_OK;
