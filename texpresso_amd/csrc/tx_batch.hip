@@ -41,9 +41,10 @@ __device__ __forceinline__ void run_chunk(const BatchTex& T, uint32_t first, uin
     }
 }
 
-// Persistent kernel: every wave takes chunks {texture, first block, count}
-// from `counter` until the queue is exhausted.  A chunk's texture fixes the
-// format and algorithm for the whole wave, so the dispatch is wave-uniform.
+// Persistent kernel: every wave takes chunks {texture, first block, count},
+// the first by its wave index and the rest from `counter`, until the queue
+// is exhausted.  A chunk's texture fixes the format and algorithm for the
+// whole wave, so the dispatch is wave-uniform.
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_batch(const BatchTex* __restrict__ tex,
                                                      const uint4* __restrict__ chunks, uint32_t nchunks,
@@ -53,11 +54,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch(const BatchTex* __restrict
     const int tid = (int)threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, l = lane & 15;
     GroupLds& L = lds[4 * wave + g];
-    for (;;) {
-        uint32_t c = 0;
-        if (lane == 0) c = atomicAdd(counter, 1u);
-        c = __builtin_amdgcn_readfirstlane(c);
-        if (c >= nchunks) break;
+    // first chunk by wave index, later ones from the counter (offset past
+    // those), so the launch does not open with one atomic per wave
+    const uint32_t nwaves = gridDim.x * WAVES;
+    for (uint32_t c = blockIdx.x * WAVES + (uint32_t)wave; c < nchunks;) {
         const uint4 ch = chunks[c];
         const BatchTex& T = tex[ch.x];
         switch (T.variant) {
@@ -73,6 +73,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch(const BatchTex* __restrict
         case 9: run_chunk<BC4, RANGE_FIT>(T, ch.y, ch.z, L, g, l); break;
         default: run_chunk<BC5, RANGE_FIT>(T, ch.y, ch.z, L, g, l); break;
         }
+        uint32_t next = 0;
+        if (lane == 0) next = atomicAdd(counter, 1u);
+        c = __builtin_amdgcn_readfirstlane(next) + nwaves;
     }
 }
 
@@ -109,21 +112,22 @@ __device__ __forceinline__ void run_chunk_light(const BatchTex& T, uint32_t firs
 
 // Persistent kernel for the alpha-only formats (BC4/BC5): no LDS and few
 // registers, so many waves stay resident to hide memory latency.  Takes
-// chunks [first, first + nchunks) through its own counter.
+// chunks [first, first + nchunks), the first by wave index and the rest
+// through its own counter.
 __global__ __launch_bounds__(256) void k_batch_light(const BatchTex* __restrict__ tex,
                                                      const uint4* __restrict__ chunks, uint32_t first,
                                                      uint32_t nchunks, uint32_t* __restrict__ counter)
 {
     const int lane = (int)threadIdx.x & 63;
-    for (;;) {
-        uint32_t c = 0;
-        if (lane == 0) c = atomicAdd(counter, 1u);
-        c = __builtin_amdgcn_readfirstlane(c);
-        if (c >= nchunks) break;
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < nchunks;) {
         const uint4 ch = chunks[first + c];
         const BatchTex& T = tex[ch.x];
         if (T.variant == 9u) run_chunk_light<BC4>(T, ch.y, ch.z, lane);
         else run_chunk_light<BC5>(T, ch.y, ch.z, lane);
+        uint32_t next = 0;
+        if (lane == 0) next = atomicAdd(counter, 1u);
+        c = __builtin_amdgcn_readfirstlane(next) + nwaves;
     }
 }
 

@@ -134,8 +134,9 @@ __device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_i
     }
 }
 
-// Persistent classify over the heavy chunks (BC1/BC2/BC3 textures).  Chunk
-// c covers blocks [first, first + count) of one texture and record slots
+// Classify over the heavy chunks (BC1/BC2/BC3 textures), each workgroup
+// taking chunks blockIdx.x, blockIdx.x + gridDim.x, ...  Chunk c covers
+// blocks [first, first + count) of one texture and record slots
 // [slot0, slot0 + count).
 __global__ __launch_bounds__(64) void k_pipe_classify(const BatchTex* __restrict__ tex, BatchTex tex0,
                                                       const uint4* __restrict__ chunks, uint32_t nchunks,
@@ -151,11 +152,9 @@ __global__ __launch_bounds__(64) void k_pipe_classify(const BatchTex* __restrict
         H.b[lane] = 0u;
     }
     group_sync();
-    for (;;) {
-        uint32_t c = 0;
-        if (lane == 0) c = atomicAdd(&ctl->chunk_counter, 1u);
-        c = __builtin_amdgcn_readfirstlane(c);
-        if (c >= nchunks) break;
+    // chunks hold similar work, so a plain grid-stride loop balances them
+    // (a shared work counter would serialise thousands of atomics at launch)
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         uint4 ch;
         if (chunks) {
             ch = chunks[c];
@@ -303,7 +302,10 @@ __global__ __launch_bounds__(64) void k_pipe_search(const BatchTex* __restrict__
     // kPull at a time with one global atomic; kPull is at most the records
     // per wave, so a short list still spreads over every wave
     const uint32_t kPull = min(16u, max(4u, total / gridDim.x));
-    uint32_t pool = 0, pool_end = 0;
+    // the first run is fixed by the workgroup index (no atomic burst at
+    // launch); refills come from the shared cursor, offset past those runs
+    uint32_t pool = blockIdx.x * kPull, pool_end = pool + kPull;
+    const uint32_t cursor_base = gridDim.x * kPull;
     for (;;) {
         // ---- ordering phase
         for (;;) {
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(64) void k_pipe_search(const BatchTex* __restrict__
                     const uint32_t grab = max(kPull, nwant);
                     uint32_t fresh = 0;
                     if (lane == 0) fresh = atomicAdd(cursor, grab);
-                    fresh = __builtin_amdgcn_readfirstlane(fresh);
+                    fresh = __builtin_amdgcn_readfirstlane(fresh) + cursor_base;
                     if (rank >= avail) p = fresh + (rank - avail);
                     pool = fresh + (nwant - avail);
                     pool_end = fresh + grab;

@@ -49,7 +49,6 @@ static_assert(offsetof(ClusterRec, remap) == 256 && offsetof(ClusterRec, alpha) 
 // is aggregated (per workgroup or per batch of records) so that no counter
 // takes one global atomic per block.
 struct PipeCtl {
-    uint32_t chunk_counter;
     uint32_t cursor_a, cursor_b;
     uint32_t hist_a[17], hist_b[17];
     uint32_t pos_a[17], pos_b[17];
