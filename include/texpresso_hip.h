@@ -304,6 +304,20 @@ int tx_dds_write_header(int format, size_t width, size_t height, uint32_t levels
 int tx_dds_parse_header(const uint8_t* data, size_t len, int* format, size_t* width, size_t* height,
                         uint32_t* levels, size_t* data_offset);
 
+/* ---- PNG image I/O of the command-line tool (host only), replacing the png
+ * crate calls of texpresso's CLI: read = cli/src/image/png.rs:59-99 (EXPAND |
+ * STRIP_16, then grey / grey-alpha / RGB / RGBA to RGBA8), write =
+ * cli/src/image/png.rs:101-111 (RGBA8, 8 bits per sample).  Buffers belong
+ * to the caller: tx_png_info gives the size, tx_png_decode fills 4*w*h
+ * bytes; tx_png_encode_bound gives a sufficient output size (0 if the image
+ * is too large) and tx_png_encode stores the file length in *out_len.
+ * Malformed files (signature, CRC, chunk order, zlib stream, filter type)
+ * return TX_ERR_FORMAT or TX_ERR_INPUT_SHORT. */
+int tx_png_info(const uint8_t* data, size_t len, size_t* width, size_t* height);
+int tx_png_decode(const uint8_t* data, size_t len, uint8_t* rgba, size_t rgba_len);
+size_t tx_png_encode_bound(size_t width, size_t height);
+int tx_png_encode(const uint8_t* rgba, size_t width, size_t height, uint8_t* out, size_t out_cap, size_t* out_len);
+
 #ifdef __cplusplus
 }
 #endif

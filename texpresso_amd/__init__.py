@@ -133,6 +133,11 @@ class Params:
 _lib = None
 
 
+def cli_path():
+    """The `texpresso` command-line tool built beside the library."""
+    return os.path.join(HERE, "texpresso")
+
+
 def library_path():
     return LIB_PATH
 
@@ -202,6 +207,10 @@ def lib():
             "tx_dds_parse_header": (c_int, [c_u8p, c_sz, ctypes.POINTER(c_int), ctypes.POINTER(c_sz),
                                             ctypes.POINTER(c_sz), ctypes.POINTER(ctypes.c_uint32),
                                             ctypes.POINTER(c_sz)]),
+            "tx_png_info": (c_int, [c_u8p, c_sz, ctypes.POINTER(c_sz), ctypes.POINTER(c_sz)]),
+            "tx_png_decode": (c_int, [c_u8p, c_sz, c_u8p, c_sz]),
+            "tx_png_encode_bound": (c_sz, [c_sz, c_sz]),
+            "tx_png_encode": (c_int, [c_u8p, c_sz, c_sz, c_u8p, c_sz, ctypes.POINTER(c_sz)]),
             "tx_error_string": (ctypes.c_char_p, [c_int]),
             "tx_device_arch": (ctypes.c_char_p, []),
         }

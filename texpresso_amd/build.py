@@ -16,7 +16,11 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtexpresso_hip.so")
 SOURCES = ["bcn_kernels.hip", "tx_batch.hip", "tx_capi.hip", "tx_context.hip", "tx_dds.hip", "tx_mips.hip",
-           "tx_pipeline.hip", "tx_probe.hip", "tx_server.hip"]
+           "tx_pipeline.hip", "tx_png.hip", "tx_probe.hip", "tx_server.hip"]
+# command-line tool over the C ABI, linked against the library with an
+# $ORIGIN run path so it finds it beside itself
+CLI_SOURCE = os.path.join(CSRC, "cli", "texpresso_cli.cpp")
+CLI = os.path.join(HERE, "texpresso")
 ARCH = "gfx950"
 
 
@@ -44,7 +48,7 @@ def flags():
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + flags() + ["-shared", "-o", tmp] + srcs
+    cmd = [hipcc()] + flags() + ["-shared", "-o", tmp] + srcs + ["-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -54,7 +58,22 @@ def build(verbose=False):
     if verbose and out.strip():
         print(out)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose=False):
+    tmp = CLI + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           CLI_SOURCE, "-o", tmp, "-L" + HERE, "-l:libtexpresso_hip.so", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + res.stdout)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":
