@@ -43,21 +43,34 @@ struct FitParams {
 
 // LDS of the front half of a block (colour set, principal axis, single and
 // range fits): the pixels and the distinct colours, 320 bytes.  The front
-// functions take any LDS type with these two members.
+// functions take any LDS type with `pts` and the put_pix / pix accessors.
 struct __align__(16) FrontLds {
-    float4 pts[16];    // distinct colours: x, y, z, sqrt(weight)
-    uint32_t pix[16];  // the block's RGBA8 pixels
+    float4 pts[16];       // distinct colours: x, y, z, sqrt(weight)
+    uint32_t pixels[16];  // the block's RGBA8 pixels
+
+    __device__ void put_pix(int l, uint32_t px) { pixels[l] = px; }
+    __device__ uint32_t pix(int m) const { return pixels[m]; }
 };
 
-// Per-group LDS (one per 4x4 block in flight), 3.2 KiB.
+// Per-group LDS (one per 4x4 block in flight), 2512 bytes, so that 16
+// single-wave workgroups (64 groups) fit in a CU's 160 KiB.  The first 80
+// bytes of the prefix table are also scratch while they hold nothing live:
+// the block's pixels at the start of a block, then, while an ordering is
+// built, the projection keys (bytes 0..63) and the ordering as bytes
+// (64..79).  Every table entry there is rewritten before it is read again.
 struct __align__(16) GroupLds {
-    float4 S[txt::kTriEntries];  // prefix sums of the weighted ordered points
-    float4 pts[16];              // distinct colours: x, y, z, sqrt(weight)
-    float4 pw[16];               // weighted points in the current order
-    uint32_t pix[16];            // the block's RGBA8 pixels
-    float keys[16];              // projections onto the ordering axis
-    uint32_t ord[8][4];          // orderings searched so far (16 bytes each)
+    float4 S[txt::kSEntries];  // prefix sums of the weighted ordered points (txt::soff layout)
+    float4 pts[16];            // distinct colours: x, y, z, sqrt(weight)
+    uint2 ord[8];              // orderings searched so far, 16 four-bit indices each
+
+    __device__ float* keys() { return reinterpret_cast<float*>(S); }
+    __device__ const float* keys() const { return reinterpret_cast<const float*>(S); }
+    __device__ uint8_t* order_bytes() { return reinterpret_cast<uint8_t*>(S + 4); }
+    // pixels kept as float bit patterns (loads and stores only) in the key area
+    __device__ void put_pix(int l, uint32_t px) { keys()[l] = __uint_as_float(px); }
+    __device__ uint32_t pix(int m) const { return __float_as_uint(keys()[m]); }
 };
+static_assert(sizeof(GroupLds) == 2512, "group LDS size");
 
 // Device copies of the constexpr tables (one per translation unit: the code
 // objects are built without relocatable device code).
@@ -571,43 +584,55 @@ __device__ __forceinline__ bool ordering_step(GroupLds& L, int l, int n, bool ac
         idx = (uint32_t)l;
     }
     if (!__builtin_isfinite(key)) key = __builtin_huge_valf();  // non-finite keys tie, after all finite
-    if (active) L.keys[l] = key;
+    float* keys = L.keys();
+    if (active) keys[l] = key;
     group_sync();
     // stable rank: entries with a smaller key, plus equal keys at lower index
     int rank = 0;
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-        const float km = L.keys[m];
+        const float km = keys[m];
         rank += (km < key) ? 1 : 0;
         rank += (m < l && km == key) ? 1 : 0;
     }
-    if (active) reinterpret_cast<uint8_t*>(L.ord[it])[rank] = (uint8_t)idx;
+    uint8_t* bytes = L.order_bytes();
+    if (active) bytes[rank] = (uint8_t)idx;
     group_sync();
+    // the ordering as 16 four-bit indices (position q in bits 4q..4q+3)
     bool unique = active;
+    uint2 cur = make_uint2(0u, 0u);
     if (active) {
-        const uint4 cur = *reinterpret_cast<const uint4*>(L.ord[it]);
+        const uint4 b = *reinterpret_cast<const uint4*>(bytes);
+        auto nib = [](uint32_t w) {
+            return (w & 0xFu) | ((w >> 4) & 0xF0u) | ((w >> 8) & 0xF00u) | ((w >> 12) & 0xF000u);
+        };
+        cur = make_uint2(nib(b.x) | (nib(b.y) << 16), nib(b.z) | (nib(b.w) << 16));
         for (int p = 0; p < it; ++p) {
-            const uint4 o = *reinterpret_cast<const uint4*>(L.ord[p]);
-            if (o.x == cur.x && o.y == cur.y && o.z == cur.z && o.w == cur.w) unique = false;
+            const uint2 o = L.ord[p];
+            if (o.x == cur.x && o.y == cur.y) unique = false;
         }
+        if (l == 0) L.ord[it] = cur;
     }
-    // weighted points in order, then S[a][b] = ((0 + pw[a]) + ...) + pw[b-1]
+    // S[a][b] = ((0 + pw[a]) + pw[a+1]) + ... + pw[b-1] with the weighted
+    // points pw in order.  S[a][a+1] = 0 + pw[a] goes in first (it equals
+    // pw[a]: every component is a product of non-negative finite values), then
+    // each lane extends its row a = l from the S[m][m+1] of the others.
     if (unique && l < n) {
-        const uint32_t o = (L.ord[it][l >> 2] >> (8 * (l & 3))) & 255u;
+        const uint32_t o = ((l < 8 ? cur.x : cur.y) >> (4 * (l & 7))) & 15u;
         const float4 q = L.pts[o];
-        L.pw[l] = make_float4(q.x * q.w, q.y * q.w, q.z * q.w, 1.0f * q.w);
+        L.S[txt::soff(l, l + 1)] =
+            make_float4(0.0f + q.x * q.w, 0.0f + q.y * q.w, 0.0f + q.z * q.w, 0.0f + 1.0f * q.w);
     }
+    if (unique && l == 0) L.S[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // the shared empty range
     group_sync();
-    if (unique) {
-        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        L.S[txt::tri(l, l)] = s;
-        if (l == 0) L.S[txt::tri(16, 16)] = s;
-        for (int t = 0; t < 16; ++t) {
-            const int bb = l + 1 + t;
+    if (unique && l < n) {
+        float4 s = L.S[txt::soff(l, l + 1)];
+        for (int t = 0; t < 15; ++t) {
+            const int bb = l + 2 + t;
             if (bb <= n) {
-                const float4 w = L.pw[bb - 1];
+                const float4 w = L.S[txt::soff(bb - 1, bb)];
                 s.x = s.x + w.x; s.y = s.y + w.y; s.z = s.z + w.z; s.w = s.w + w.w;
-                L.S[txt::tri(l, bb)] = s;
+                L.S[txt::soff(l, bb)] = s;
             }
         }
     }
@@ -714,12 +739,14 @@ __device__ __forceinline__ uint32_t split_index(const GroupLds& L, int n, int re
 {
     if (remap < 0) return 3u;
     const int i = (int)((code >> 16) & 15u);
-    const int j = (int)(code & 255u) - txt::tri(i, 0);
-    const int k = NC == 4 ? (int)((code >> 8) & 255u) - txt::tri(j, 0) : 0;
+    const uint32_t o1 = code & 255u, o2 = (code >> 8) & 255u;
+    const int j = o1 ? (int)c_cand.off_b[o1] : i;
+    const int k = NC == 4 ? (o2 ? (int)c_cand.off_b[o2] : j) : 0;
+    const uint2 ord = L.ord[it];
     int qmax = -1;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int pq = (int)((L.ord[it][q >> 2] >> (8 * (q & 3))) & 255u);
+        const int pq = (int)(((q < 8 ? ord.x : ord.y) >> (4 * (q & 7))) & 15u);
         if (q >= i && q < n && pq == remap) qmax = q;
     }
     if (qmax < 0) return 0u;
@@ -940,12 +967,12 @@ __device__ __forceinline__ int encode_front(uint32_t px, bool valid, const FitPa
     const bool inc = valid && !(FMT == BC1 && alpha < 128u);
     const uint32_t incmask = (uint32_t)(__ballot(inc) >> gshift) & 0xFFFFu;
     cs.transparent = FMT == BC1 && ((uint32_t)(__ballot(valid && alpha < 128u) >> gshift) & 0xFFFFu) != 0;
-    L.pix[l] = px;
+    L.put_pix(l, px);
     group_sync();
     uint32_t same = 0, asum = 0;
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-        const uint32_t pm = L.pix[m];
+        const uint32_t pm = L.pix(m);
         const bool s = ((incmask >> m) & 1u) && ((pm & 0xFFFFFFu) == rgb);
         same |= (s ? 1u : 0u) << m;
         asum += s ? (pm >> 24) + 1u : 0u;

@@ -171,7 +171,7 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
     p.tex0.wb = a.wb;
     p.tex0.weigh_alpha = a.weigh_alpha;
     // persistent single-wave workgroups: classify fits 20 per CU (320 B of
-    // LDS per block), the searches 12 (3.2 KiB per block)
+    // LDS per block), the searches 16 (2512 B per block)
     const uint64_t cwaves = (uint64_t)ctx->cus * txp::kClassifyWavesPerCu;
     const uint64_t swaves = (uint64_t)ctx->cus * txp::kSearchWavesPerCu;
     uint64_t cb = (a.nblocks + cwaves - 1) / cwaves;

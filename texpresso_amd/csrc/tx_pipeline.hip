@@ -281,7 +281,7 @@ __device__ __forceinline__ void emit_block(const BatchTex* __restrict__ tex, con
 // candidate splits; the candidate loop runs as long as the largest colour
 // count among them needs, and the records are sorted by colour count.
 template <int NC>
-__global__ __launch_bounds__(64) void k_pipe_search(const BatchTex* __restrict__ tex, BatchTex tex0,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_search(const BatchTex* __restrict__ tex, BatchTex tex0,
                                                     ClusterRec* __restrict__ recs, const uint32_t* __restrict__ list,
                                                     const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor)
 {

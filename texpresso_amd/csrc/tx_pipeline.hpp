@@ -69,9 +69,10 @@ struct PipeBufs {
 };
 
 // Resident single-wave workgroups per CU: classify is limited by registers
-// (5 waves per SIMD), the searches by LDS (4 x 3.2 KiB per workgroup).
+// (5 waves per SIMD), the searches by LDS and registers (4 x 2512 B per
+// workgroup, 4 waves per SIMD).
 constexpr uint32_t kClassifyWavesPerCu = 20;
-constexpr uint32_t kSearchWavesPerCu = 12;
+constexpr uint32_t kSearchWavesPerCu = 16;
 
 struct PipeLaunch {
     const txk::BatchTex* d_tex;  // texture descriptors (null: only `tex0`)

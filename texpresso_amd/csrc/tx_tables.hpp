@@ -22,18 +22,24 @@
 //    length len[n].  Each entry packs the prefix-table offsets the split
 //    reads and its rank in the reference's loop order, which breaks error
 //    ties exactly like the reference's "first strict minimum wins":
-//      bits  0..7   tri(i, j)   (compress3: part1 offset; compress4: part1)
-//      bits  8..15  tri(j, k)   (compress4 only: part2 offset)
+//      bits  0..7   soff(i, j)  (compress3: part1 offset; compress4: part1)
+//      bits  8..15  soff(j, k)  (compress4 only: part2 offset)
 //      bits 16..19  i           (part0 = S[0][i] lives at offset i)
 //      bits 20..29  loop-order rank
-//    tri(a, b) = a*(33-a)/2 + b indexes the upper triangle a <= b <= 16.
+//    The prefix table S[a][b], a <= b <= 16, is stored compactly: every
+//    empty range S[a][a] (all zero) shares offset 0, row 0 (S[0][b], b >= 1)
+//    sits at offsets 1..16 so that part0 = S[0][i] is at offset i, and rows
+//    a = 1..15 (b = a+1..16) follow: 137 entries in all.
 #pragma once
 #include <stdint.h>
 
 namespace txt {
 
-constexpr int tri(int a, int b) { return ((a * (33 - a)) >> 1) + b; }
-constexpr int kTriEntries = 153;  // tri(16, 16) + 1
+constexpr int soff(int a, int b)
+{
+    return a == b ? 0 : a == 0 ? b : 17 + 16 * (a - 1) - (((a - 1) * a) >> 1) + (b - a - 1);
+}
+constexpr int kSEntries = 137;  // soff(15, 16) + 1
 constexpr int kCand4 = 967;       // compress4 splits for 16 colours
 constexpr int kCand3 = 151;       // compress3 splits for 16 colours
 // Stored lengths: the candidate loop reads up to 48 entries past its last
@@ -51,6 +57,7 @@ struct CandTables {
     uint32_t t3[kCand3Stored];
     uint16_t len4[17];
     uint16_t len3[17];
+    uint8_t off_b[kSEntries + 3];  // b of the entry at each offset (0 for the shared zero)
 };
 
 constexpr void build_lut_one(int bits, int colours, uint8_t (&out)[256][6])
@@ -126,7 +133,7 @@ constexpr CandTables make_cand_tables()
         for (int i = 0; i < 16; ++i)
             for (int j = i; j <= 16; ++j)
                 for (int k = (j > 1 ? j : 1); k <= 16; ++k) {
-                    lex[c] = (uint32_t)tri(i, j) | ((uint32_t)tri(j, k) << 8) | ((uint32_t)i << 16) |
+                    lex[c] = (uint32_t)soff(i, j) | ((uint32_t)soff(j, k) << 8) | ((uint32_t)i << 16) |
                              ((uint32_t)c << 20);
                     bound[c] = (i + 1 > k) ? i + 1 : k;
                     ++c;
@@ -144,7 +151,7 @@ constexpr CandTables make_cand_tables()
         int c = 0;
         for (int i = 0; i < 16; ++i)
             for (int j = (i > 1 ? i : 1); j <= 16; ++j) {
-                lex[c] = (uint32_t)tri(i, j) | ((uint32_t)i << 16) | ((uint32_t)c << 20);
+                lex[c] = (uint32_t)soff(i, j) | ((uint32_t)i << 16) | ((uint32_t)c << 20);
                 bound[c] = (i + 1 > j) ? i + 1 : j;
                 ++c;
             }
@@ -155,7 +162,16 @@ constexpr CandTables make_cand_tables()
             T.len3[m] = (uint16_t)pos;
         }
     }
+    for (int a = 0; a <= 16; ++a)
+        for (int b = a + 1; b <= 16; ++b) T.off_b[soff(a, b)] = (uint8_t)b;
     return T;
+}
+
+// j (or k) of a packed candidate from the offset of S[i][j] (S[j][k]): an
+// empty range (offset 0) ends where it starts.
+constexpr int split_end(const CandTables& T, uint32_t off, int start)
+{
+    return off == 0 ? start : (int)T.off_b[off];
 }
 
 // 3. Unit-weight split constants.  When a block has 16 distinct colours and
@@ -199,8 +215,7 @@ constexpr UnitSplit unit_split3(int i, int j)
 }
 
 // (i, j, k) of a packed candidate: i from bits 16..19, j and k from the
-// prefix-table offsets tri(i, j) and tri(j, k).
-constexpr int cand_j(uint32_t code, int i) { return (int)(code & 255u) - tri(i, 0); }
+// prefix-table offsets soff(i, j) and soff(j, k).
 
 constexpr UnitTables make_unit_tables()
 {
@@ -209,16 +224,18 @@ constexpr UnitTables make_unit_tables()
     for (int e = 0; e < kCand4; ++e) {
         const uint32_t code = C.t4[e];
         const int i = (int)((code >> 16) & 15u);
-        const int j = cand_j(code, i);
-        const int k = (int)((code >> 8) & 255u) - tri(j, 0);
+        const int j = split_end(C, code & 255u, i);
+        const int k = split_end(C, (code >> 8) & 255u, j);
         U.u4[e] = unit_split4(i, j, k);
     }
     for (int e = 0; e < kCand3; ++e) {
         const uint32_t code = C.t3[e];
         const int i = (int)((code >> 16) & 15u);
-        U.u3[e] = unit_split3(i, cand_j(code, i));
+        U.u3[e] = unit_split3(i, split_end(C, code & 255u, i));
     }
     return U;
 }
+
+static_assert(soff(0, 16) == 16 && soff(1, 2) == 17 && soff(15, 16) == kSEntries - 1, "compact prefix table");
 
 }  // namespace txt
