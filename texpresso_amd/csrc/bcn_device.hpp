@@ -550,7 +550,9 @@ __device__ __forceinline__ float eval_split(const SplitSums& s, const float4& xs
         const float e1 = (av * av) * a2 + ((bv * bv) * b2);
         const float e2 = ((av * bv) * ab) - av * ax[c];
         const float e3 = e2 - bv * bx[c];
-        const float e4 = 2.0f * e3 + e1;
+        // 2 * e3 is exact (a power-of-two scaling of a small finite value),
+        // so the fused form rounds once, exactly like the product-then-add
+        const float e4 = __builtin_fmaf(2.0f, e3, e1);
         e5[c] = e4 * wts[c];
     }
     return (e5[0] + e5[1]) + e5[2];
@@ -680,10 +682,9 @@ __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t m
         if (t < mylen) {
             float a[3], b[3];
             const float err = eval_split<NC, UNIT>(sA, xs, fp, a, b, uA);
-            if (err < be || (err == be && cA < bc)) {
-                be = err;
-                bc = cA;
-            }
+            const bool better = err < be || (err == be && cA < bc);
+            be = better ? err : be;
+            bc = better ? cA : bc;
         }
         const uint32_t cB2 = code_at(ot + 192);
         sA = load_split<NC>(L, cA2);
@@ -691,10 +692,9 @@ __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t m
         if (t + 16 < mylen) {
             float a[3], b[3];
             const float err = eval_split<NC, UNIT>(sB, xs, fp, a, b, uB);
-            if (err < be || (err == be && cB < bc)) {
-                be = err;
-                bc = cB;
-            }
+            const bool better = err < be || (err == be && cB < bc);
+            be = better ? err : be;
+            bc = better ? cB : bc;
         }
         cA = cA2;
         cB = cB2;
