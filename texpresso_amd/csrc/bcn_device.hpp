@@ -807,19 +807,22 @@ __device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, in
     }
 }
 
-// alpha.rs:187-256 for a whole block on one lane (BC4 / BC5 channels): the
-// same arithmetic as bc3_alpha, with the 16 pixels of the block in `v` and
-// their mask bits in `valid`.
-__device__ __forceinline__ void bc3_alpha_lane(const int (&v)[16], uint32_t valid, uint32_t& lo, uint32_t& hi)
+// alpha.rs:187-256 for a whole block on one lane: the same arithmetic as
+// bc3_alpha, with v(p) the value of pixel p and `valid` the mask bits.  U is
+// the unroll factor of the pixel loops (16 for values in registers; a rolled
+// loop over values read from memory keeps register use down).
+template <int U, class Src>
+__device__ __forceinline__ void bc3_alpha_core(const Src& v, uint32_t valid, uint32_t& lo, uint32_t& hi)
 {
     int min7 = 255, max7 = 0, min5 = 255, max5 = 0;
-#pragma unroll
+#pragma unroll U
     for (int p = 0; p < 16; ++p) {
         if ((valid >> p) & 1u) {
-            min7 = min(min7, v[p]);
-            max7 = max(max7, v[p]);
-            if (v[p] != 0) min5 = min(min5, v[p]);
-            if (v[p] != 255) max5 = max(max5, v[p]);
+            const int vp = v(p);
+            min7 = min(min7, vp);
+            max7 = max(max7, vp);
+            if (vp != 0) min5 = min(min5, vp);
+            if (vp != 255) max5 = max(max5, vp);
         }
     }
     if (min5 > max5) min5 = max5;
@@ -840,12 +843,13 @@ __device__ __forceinline__ void bc3_alpha_lane(const int (&v)[16], uint32_t vali
     // 3-bit indices of both codebooks, pixel p at bits 3p; masked pixels 0
     uint64_t i5 = 0, i7 = 0;
     uint32_t err5 = 0, err7 = 0;
-#pragma unroll
+#pragma unroll U
     for (int p = 0; p < 16; ++p) {
         if ((valid >> p) & 1u) {
+            const int vp = v(p);
             uint32_t l5, l7;
-            i5 |= (uint64_t)fit8(v[p], codes5, l5) << (3 * p);
-            i7 |= (uint64_t)fit8(v[p], codes7, l7) << (3 * p);
+            i5 |= (uint64_t)fit8(vp, codes5, l5) << (3 * p);
+            i7 |= (uint64_t)fit8(vp, codes7, l7) << (3 * p);
             err5 += l5;
             err7 += l7;
         }
@@ -855,7 +859,7 @@ __device__ __forceinline__ void bc3_alpha_lane(const int (&v)[16], uint32_t vali
     if (err5 <= err7) {  // write_alpha_block5, alpha.rs:146-165
         if (min5 > max5) {
             a0 = max5; a1 = min5;
-#pragma unroll
+#pragma unroll U
             for (int p = 0; p < 16; ++p) {
                 const uint32_t i = (uint32_t)(i5 >> (3 * p)) & 7u;
                 const uint32_t o = i == 0 ? 1u : i == 1 ? 0u : (i >= 2 && i <= 5) ? 7u - i : i;
@@ -867,7 +871,7 @@ __device__ __forceinline__ void bc3_alpha_lane(const int (&v)[16], uint32_t vali
     } else {  // write_alpha_block7, alpha.rs:167-185
         if (min7 < max7) {
             a0 = max7; a1 = min7;
-#pragma unroll
+#pragma unroll U
             for (int p = 0; p < 16; ++p) {
                 const uint32_t i = (uint32_t)(i7 >> (3 * p)) & 7u;
                 const uint32_t o = i == 0 ? 1u : i == 1 ? 0u : 9u - i;
@@ -880,6 +884,12 @@ __device__ __forceinline__ void bc3_alpha_lane(const int (&v)[16], uint32_t vali
     // alpha.rs:120-144: the 48 index bits follow the two endpoint bytes
     lo = (uint32_t)a0 | ((uint32_t)a1 << 8) | ((uint32_t)(idx & 0xFFFFu) << 16);
     hi = (uint32_t)(idx >> 16);
+}
+
+// The BC4 / BC5 channel form: the 16 values in registers.
+__device__ __forceinline__ void bc3_alpha_lane(const int (&v)[16], uint32_t valid, uint32_t& lo, uint32_t& hi)
+{
+    bc3_alpha_core<16>([&](int p) { return v[p]; }, valid, lo, hi);
 }
 
 // One block per lane: the 16 pixels of block (bx, by) as 16 dword loads

@@ -170,12 +170,16 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
     p.tex0.wg = a.wg;
     p.tex0.wb = a.wb;
     p.tex0.weigh_alpha = a.weigh_alpha;
-    // persistent single-wave workgroups: classify fits 20 per CU (320 B of
-    // LDS per block), the searches 16 (2512 B per block)
+    // persistent single-wave workgroups, 16 per CU for both passes.  Classify
+    // runs one block per lane: chunks of 64 blocks fill a wave; a texture
+    // with fewer than 64 blocks per SIMD is cut into chunks of at least 16
+    // so that it still reaches every SIMD (one wave per SIMD with all lanes
+    // busy beats several part-filled ones)
     const uint64_t cwaves = (uint64_t)ctx->cus * txp::kClassifyWavesPerCu;
     const uint64_t swaves = (uint64_t)ctx->cus * txp::kSearchWavesPerCu;
-    uint64_t cb = (a.nblocks + cwaves - 1) / cwaves;
-    cb = std::min<uint64_t>(64, std::max<uint64_t>(4, (cb + 3) & ~(uint64_t)3));
+    const uint64_t simds = (uint64_t)ctx->cus * 4;
+    uint64_t cb = (a.nblocks + simds - 1) / simds;
+    cb = std::min<uint64_t>(64, std::max<uint64_t>(16, (cb + 15) & ~(uint64_t)15));
     p.chunk_blocks = (uint32_t)cb;
     p.nchunks = (uint32_t)((a.nblocks + cb - 1) / cb);
     p.nslots = a.nblocks;

@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "bcn_device.hpp"
+#include "bcn_lane.hpp"
 #include "tx_pipeline.hpp"
 
 #pragma clang fp contract(off)
@@ -58,79 +59,67 @@ struct ClassifyHist {
     uint32_t a[17], b[17];
 };
 
-// ClusterFit block: write its record into slot r and count it into the
-// workgroup's bins.  Lane l stores point l; lanes 0-4 store the 80-byte
-// tail (every value there is the same on all lanes of the group).
+// ClusterFit block on one lane: its record into slot r, its key, and its
+// count in the workgroup's bins.
 template <int FMT, int ALG>
-__device__ __forceinline__ void write_record(ClusterRec* __restrict__ recs, uint32_t r, ClassifyHist& H,
-                                             const FrontLds& L, int l, const ColourSet& cs, const uint32_t (&out)[4],
-                                             float vx, float vy, float vz, const FitParams& fp, uint32_t tex_idx,
-                                             uint32_t block, uint32_t& key)
+__device__ __forceinline__ uint32_t write_record_lane(ClusterRec* __restrict__ recs, uint32_t r, ClassifyHist& H,
+                                                      const LaneSet& S, const LaneTables& T, const uint32_t (&out)[4],
+                                                      float vx, float vy, float vz, const FitParams& fp,
+                                                      uint32_t tex_idx, uint32_t block)
 {
     ClusterRec& R = recs[r];
-    R.pts[l] = L.pts[l];
-    // one byte per pixel: word q holds pixels 4q..4q+3
-    const uint32_t byte = (uint32_t)(cs.remap < 0 ? 255 : cs.remap) << (8 * (l & 3));
-    const uint32_t r0 = grp_or((l >> 2) == 0 ? byte : 0u);
-    const uint32_t r1 = grp_or((l >> 2) == 1 ? byte : 0u);
-    const uint32_t r2 = grp_or((l >> 2) == 2 ? byte : 0u);
-    const uint32_t r3 = grp_or((l >> 2) == 3 ? byte : 0u);
-    const bool need4 = !(FMT == BC1 && cs.transparent);
-    const uint32_t info = (uint32_t)cs.n | ((cs.transparent ? 1u : 0u) << 8) | ((uint32_t)FMT << 12) |
+#pragma unroll 2
+    for (int p = 0; p < 16; ++p) {
+        if ((S.first >> p) & 1u) {
+            float x, y, z;
+            xyz(T, S.pix(p), x, y, z);
+            R.pts[__popc(S.first & ((1u << p) - 1u))] = make_float4(x, y, z, S.wt(p));
+        }
+    }
+    const bool need4 = !(FMT == BC1 && S.transparent);
+    const uint32_t info = (uint32_t)S.n | ((S.transparent ? 1u : 0u) << 8) | ((uint32_t)FMT << 12) |
                           ((ALG == ITERATIVE_CLUSTER_FIT ? 8u : 1u) << 16);
-    uint4 v;
-    switch (l) {
-    case 0: v = make_uint4(r0, r1, r2, r3); break;
-    case 1: v = make_uint4(out[0], out[1], __float_as_uint(vx), __float_as_uint(vy)); break;
-    case 2: v = make_uint4(__float_as_uint(vz), __float_as_uint(FLT_MAX), 0u, 0u); break;
-    case 3: v = make_uint4(__float_as_uint(fp.wr), __float_as_uint(fp.wg), __float_as_uint(fp.wb), tex_idx); break;
-    default: v = make_uint4(block, info, 0u, 0u); break;
-    }
-    if (l < 5) reinterpret_cast<uint4*>(R.remap)[l] = v;
-    if (l == 0) {
-        key = (uint32_t)cs.n | (FMT == BC1 ? kKeyA : 0u) | (need4 ? kKeyB : 0u);
-        if (FMT == BC1) atomicAdd(&H.a[cs.n], 1u);
-        if (need4) atomicAdd(&H.b[cs.n], 1u);
-    }
+    uint4* tail = reinterpret_cast<uint4*>(R.remap);
+    tail[0] = make_uint4(S.remap[0], S.remap[1], S.remap[2], S.remap[3]);
+    tail[1] = make_uint4(out[0], out[1], __float_as_uint(vx), __float_as_uint(vy));
+    tail[2] = make_uint4(__float_as_uint(vz), __float_as_uint(FLT_MAX), 0u, 0u);
+    tail[3] = make_uint4(__float_as_uint(fp.wr), __float_as_uint(fp.wg), __float_as_uint(fp.wb), tex_idx);
+    tail[4] = make_uint4(block, info, 0u, 0u);
+    if (FMT == BC1) atomicAdd(&H.a[S.n], 1u);
+    if (need4) atomicAdd(&H.b[S.n], 1u);
+    return (uint32_t)S.n | (FMT == BC1 ? kKeyA : 0u) | (need4 ? kKeyB : 0u);
 }
 
-// Blocks [first, first + count) of one texture, four at a time; the next
-// four blocks' pixels are loaded before the current four are classified.
+// Blocks [first, first + count) of one texture, one per lane, 64 at a time.
+// Single-colour and range-fit blocks are finished here; ClusterFit blocks
+// become records in slots [slot0, slot0 + count).
 template <int FMT, int ALG>
 __device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_idx, uint32_t first, uint32_t count,
-                                               uint32_t slot0, FrontLds& L, int g, int l, ClusterRec* recs,
-                                               uint32_t* keys, ClassifyHist& H)
+                                               uint32_t slot0, const LaneTables& LT, LaneBlockLds& LB, int lane,
+                                               ClusterRec* recs, uint32_t* keys, ClassifyHist& H)
 {
     const FitParams fp{T.wr, T.wg, T.wb, T.weigh_alpha};
-    auto fetch = [&](uint32_t off, bool& valid) {
-        const uint32_t b = first + off + (uint32_t)g;
-        const bool live = off + (uint32_t)g < count;
-        return fetch_pixel(T.rgba, T.width, T.height, live ? b % T.blocks_wide : 0u, live ? b / T.blocks_wide : 0u,
-                           live, l, valid);
-    };
-    bool v_next;
-    uint32_t px_next = fetch(0, v_next);
-    for (uint32_t off = 0; off < count; off += 4) {
-        const uint32_t b = first + off + (uint32_t)g;
-        const bool live = off + (uint32_t)g < count;
-        const bool valid = v_next;
-        const uint32_t px = px_next;
-        px_next = fetch(off + 4, v_next);
-        uint32_t w[4];
-        ColourSet cs;
-        float vx, vy, vz;
-        const int kind = encode_front<FMT, ALG>(px, valid, fp, L, g, l, w, cs, vx, vy, vz);
-        const bool cluster = live && kind == KIND_CLUSTER;
-        if (live && !cluster && l == 0) {
-            if (FMT == BC1)
+    for (uint32_t off = 0; off < count; off += 64) {
+        const uint32_t k = off + (uint32_t)lane;
+        if (k < count) {
+            const uint32_t b = first + k;
+            uint32_t px[16];
+            const uint32_t valid = fetch_block_lane(T.rgba, T.width, T.height, b % T.blocks_wide,
+                                                    b / T.blocks_wide, px);
+            uint32_t w[4];
+            LaneSet S;
+            float vx, vy, vz;
+            const int kind = encode_front_lane<FMT, ALG>(px, valid, fp, LT, LB, lane, w, S, vx, vy, vz);
+            uint32_t key = 0;
+            if (kind == KIND_CLUSTER) {
+                key = write_record_lane<FMT, ALG>(recs, slot0 + k, H, S, LT, w, vx, vy, vz, fp, tex_idx, b);
+            } else if (FMT == BC1) {
                 *reinterpret_cast<uint2*>(T.out + (size_t)b * 8u) = make_uint2(w[0], w[1]);
-            else
+            } else {
                 *reinterpret_cast<uint4*>(T.out + (size_t)b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+            keys[slot0 + k] = key;
         }
-        uint32_t key = 0;
-        if (cluster) write_record<FMT, ALG>(recs, slot0 + off + (uint32_t)g, H, L, l, cs, w, vx, vy, vz, fp, tex_idx, b,
-                                            key);
-        if (live && l == 0) keys[slot0 + off + (uint32_t)g] = key;
     }
 }
 
@@ -138,15 +127,16 @@ __device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_i
 // taking chunks blockIdx.x, blockIdx.x + gridDim.x, ...  Chunk c covers
 // blocks [first, first + count) of one texture and record slots
 // [slot0, slot0 + count).
-__global__ __launch_bounds__(64) void k_pipe_classify(const BatchTex* __restrict__ tex, BatchTex tex0,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_classify(const BatchTex* __restrict__ tex, BatchTex tex0,
                                                       const uint4* __restrict__ chunks, uint32_t nchunks,
                                                       uint32_t chunk_blocks, ClusterRec* __restrict__ recs,
                                                       uint32_t* __restrict__ keys, PipeCtl* __restrict__ ctl)
 {
-    __shared__ FrontLds lds[4];
+    __shared__ LaneTables LT;
+    __shared__ LaneBlockLds LB;
     __shared__ ClassifyHist H;
-    const int lane = (int)threadIdx.x, g = lane >> 4, l = lane & 15;
-    FrontLds& L = lds[g];
+    const int lane = (int)threadIdx.x;
+    fill_lane_tables(LT, lane);
     if (lane < 17) {
         H.a[lane] = 0u;
         H.b[lane] = 0u;
@@ -164,15 +154,15 @@ __global__ __launch_bounds__(64) void k_pipe_classify(const BatchTex* __restrict
         }
         const BatchTex& T = tex_at(tex, tex0, ch.x);
         switch (T.variant) {
-        case 0: classify_chunk<BC1, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
-        case 1: classify_chunk<BC1, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
-        case 2: classify_chunk<BC1, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
-        case 3: classify_chunk<BC2, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
-        case 4: classify_chunk<BC2, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
-        case 5: classify_chunk<BC2, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
-        case 6: classify_chunk<BC3, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
-        case 7: classify_chunk<BC3, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
-        default: classify_chunk<BC3, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, L, g, l, recs, keys, H); break;
+        case 0: classify_chunk<BC1, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
+        case 1: classify_chunk<BC1, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
+        case 2: classify_chunk<BC1, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
+        case 3: classify_chunk<BC2, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
+        case 4: classify_chunk<BC2, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
+        case 5: classify_chunk<BC2, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
+        case 6: classify_chunk<BC3, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
+        case 7: classify_chunk<BC3, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
+        default: classify_chunk<BC3, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
         }
     }
     // one flush of the workgroup's bin counts

@@ -2,9 +2,9 @@
 //
 //   reset -> classify -> bin -> search<3> (BC1) -> search<4>
 //
-// classify: alpha, colour set and fit choice for every block; single-colour
-//   and range-fit blocks are finished here, ClusterFit blocks become a
-//   ClusterRec and are counted by distinct-colour count n.
+// classify: alpha, colour set and fit choice for every block, one block per
+//   lane; single-colour and range-fit blocks are finished here, ClusterFit
+//   blocks become a ClusterRec and are counted by distinct-colour count n.
 // bin: counting sort of the records by n (16 first) into one list per pass
 //   (list A: BC1 records for the 3-colour search; list B: records that need
 //   the 4-colour search).
@@ -68,10 +68,11 @@ struct PipeBufs {
     void carve(uint8_t* base, uint64_t cap);
 };
 
-// Resident single-wave workgroups per CU: classify is limited by registers
-// (5 waves per SIMD), the searches by LDS and registers (4 x 2512 B per
-// workgroup, 4 waves per SIMD).
-constexpr uint32_t kClassifyWavesPerCu = 20;
+// Resident single-wave workgroups per CU: classify (one block per lane) is
+// limited by registers and LDS (8 KiB of pixel and weight copies per
+// workgroup), the searches by LDS and registers (4 x 2512 B per workgroup);
+// both fit 4 waves per SIMD.
+constexpr uint32_t kClassifyWavesPerCu = 16;
 constexpr uint32_t kSearchWavesPerCu = 16;
 
 struct PipeLaunch {
