@@ -68,7 +68,7 @@ bool DeviceContext::pipe_scratch(uint64_t recs, txp::PipeBufs* out)
 }
 
 namespace {
-std::atomic<uint64_t> g_pipe_min_blocks{131072};
+std::atomic<uint64_t> g_pipe_min_blocks{45056};
 }
 
 uint64_t pipeline_min_blocks() { return g_pipe_min_blocks.load(); }
