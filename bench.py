@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BC1/BC3 ClusterFit Mpix/s on 1 MI355X (bit-exact) vs host CPU; p99 job latency"
 VALU_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (FMA = 2 flops, packed)
-VALU_PLAIN_TOPS = 39.3       # one plain FP32 op (mul/add/fma) per lane per clock: 256 CU x 64 lanes x 2.4 GHz
+VALU_PLAIN_TOPS = 78.6       # plain FP32 mul/add with vector-register operands: 1024 SIMDs x 32 lanes x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
 
 

@@ -54,6 +54,39 @@ def test_pipeline_adversarial_blocks_as_image(pipeline_always, oracle, fmt, alph
     assert np.array_equal(got, want)
 
 
+def test_pipeline_classify_every_fit_on_adversarial_blocks(gpu, oracle):
+    # every format x algorithm x alpha weighting in one pipeline batch, so
+    # the one-block-per-lane classify takes each fit path (single colour,
+    # range fit, ClusterFit records, transparent BC1, NaN axes, ties) and
+    # RangeFit textures (which single-texture calls never send here) as well
+    items = []
+    for fmt in (0, 1, 2):
+        blocks, _ = make_blocks(301 + fmt, per_class=24)
+        nb = len(blocks) - len(blocks) % 16
+        img = np.ascontiguousarray(blocks[:nb].reshape(nb // 16, 16, 4, 4, 4).transpose(0, 2, 1, 3, 4)
+                                   .reshape(nb // 16 * 4, 64, 4))
+        for alg in (0, 1, 2):
+            for alpha in (False, True):
+                items.append((fmt, tx.Params(tx.Algorithm(alg), (0.9, 0.05, 0.37), alpha), img))
+    bufs, descs = [], []
+    for fmt, p, img in items:
+        h, w = img.shape[:2]
+        d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
+        n = tx.Format(fmt).compressed_size(w, h)
+        d_out = tx.DeviceBuffer(n)
+        bufs.append((d_in, d_out, n))
+        descs.append((fmt, p, d_in.ptr, d_out.ptr, w, h))
+    batch = tx.Batch(descs, tx.Tuning(chunk_blocks=64, waves_per_group=1, groups_per_cu=12, use_graph=False,
+                                      pipeline=True))
+    batch.run()
+    for (fmt, p, img), (_, d_out, n) in zip(items, bufs):
+        h, w = img.shape[:2]
+        op = oracle.params(int(p.algorithm), p.weights, p.weigh_colour_by_alpha)
+        want = oracle.compress(fmt, img, w, h, op, threads=oracle.default_threads())
+        assert np.array_equal(d_out.download(n), want), (fmt, p.algorithm, p.weigh_colour_by_alpha)
+    batch.close()
+
+
 def test_pipeline_oversized_output(pipeline_always, oracle):
     img = texgen.rgb_texture(9, 6, 4)
     n = tx.Format.Bc1.compressed_size(9, 6) + 5 * 8

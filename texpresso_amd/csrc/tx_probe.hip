@@ -1,6 +1,9 @@
 // tx_probe.hip -- microbenchmark of the plain FP32 VALU rate (separate
 // multiply and add, no FMA, no packed math: the only FP32 forms the
 // bit-exact encoder can use).  Used to state the encoder's compute ceiling.
+// The operands are per-lane values in vector registers, as in the encoder's
+// split loop: an operand in a scalar register (a kernel argument used
+// directly) halves the issue rate of these instructions on this GPU.
 #include <hip/hip_runtime.h>
 
 #include "texpresso_hip.h"
@@ -18,13 +21,16 @@ __global__ __launch_bounds__(256) void k_valu_probe(float* out, int iters, float
     float x[kChains];
 #pragma unroll
     for (int c = 0; c < kChains; ++c) x[c] = (float)(threadIdx.x + c) * 1e-3f;
+    // lane-dependent copies of the factors keep them in vector registers
+    const float av = a + (float)(threadIdx.x & 1u) * 1e-30f;
+    const float bv = b + (float)(threadIdx.x & 2u) * 1e-30f;
     for (int i = 0; i < iters; ++i) {
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
 #pragma unroll
-            for (int c = 0; c < kChains; ++c) x[c] = x[c] * a;
+            for (int c = 0; c < kChains; ++c) x[c] = x[c] * av;
 #pragma unroll
-            for (int c = 0; c < kChains; ++c) x[c] = x[c] + b;
+            for (int c = 0; c < kChains; ++c) x[c] = x[c] + bv;
         }
     }
     float s = 0.0f;
