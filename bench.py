@@ -421,6 +421,8 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
+    ap.add_argument("--valu-sweep", action="store_true",
+                    help="print the plain FP32 VALU issue rate at 1..8 waves per SIMD and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     ap.add_argument("--suite", default="", help="comma list of cfg0,cfg2,cfg3: per-config lines instead of the "
@@ -440,6 +442,15 @@ def main():
         if args.gpus != 1:
             sys.exit(2)
         run_suite(args)
+        return
+    if args.valu_sweep:
+        import ctypes
+        import texpresso_amd as tx
+        rate = ctypes.c_double()
+        for w in range(1, 9):
+            if tx.lib().tx_util_valu_probe_waves(w, ctypes.byref(rate)) != 0:
+                sys.exit("valu probe failed")
+            print(json.dumps({"waves_per_simd": w, "plain_fp32_tops": round(rate.value / 1e12, 2)}), flush=True)
         return
     if args.copy_calibration:
         import texpresso_amd as tx

@@ -251,9 +251,12 @@ int tx_util_time_phases(int format, const uint8_t* d_rgba, size_t width, size_t 
  * Process-wide; returns the previous value (default 45056). */
 uint64_t tx_set_pipeline_threshold(uint64_t min_blocks);
 
-/* Measures the device's plain FP32 VALU rate (separate multiply and add, no
- * FMA, no packed math) in lane-operations per second. */
+/* Measures the device's plain FP32 VALU rate (separate multiply and add with
+ * vector-register operands, no FMA, no packed math) in lane-operations per
+ * second, with 8 waves per SIMD; the _waves form runs 1..8 waves per SIMD
+ * (the issue rate grows with the waves a SIMD can choose from). */
 int tx_util_valu_probe(double* lane_ops_per_s);
+int tx_util_valu_probe_waves(int waves_per_simd, double* lane_ops_per_s);
 
 /* Human-readable text for an error code. */
 const char* tx_error_string(int code);

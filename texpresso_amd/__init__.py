@@ -170,6 +170,7 @@ def lib():
             "tx_util_download": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_copy": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_valu_probe": (c_int, [ctypes.POINTER(ctypes.c_double)]),
+            "tx_util_valu_probe_waves": (c_int, [c_int, ctypes.POINTER(ctypes.c_double)]),
             "tx_set_pipeline_threshold": (ctypes.c_uint64, [ctypes.c_uint64]),
             "tx_util_time_compress": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
                                               ctypes.POINTER(ctypes.c_float)]),

@@ -41,16 +41,18 @@ __global__ __launch_bounds__(256) void k_valu_probe(float* out, int iters, float
 
 }  // namespace
 
-extern "C" int tx_util_valu_probe(double* lane_ops_per_s)
+extern "C" int tx_util_valu_probe_waves(int waves_per_simd, double* lane_ops_per_s)
 {
-    if (!lane_ops_per_s) return TX_ERR_INVALID;
+    if (!lane_ops_per_s || waves_per_simd < 1 || waves_per_simd > 8) return TX_ERR_INVALID;
     int rc = TX_OK;
     txc::DeviceContext* ctx = txc::current_context(&rc);
     if (!ctx) return rc;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return TX_ERR_HIP;
-    const int blocks = prop.multiProcessorCount * 8;  // 8 x 256 threads per CU
-    const int threads = 256, iters = 2048;
+    // workgroups of 4 waves (one per SIMD of a CU), `waves_per_simd` of them
+    // per CU; fewer waves get proportionally more iterations
+    const int blocks = prop.multiProcessorCount * waves_per_simd;
+    const int threads = 256, iters = 2048 * 8 / waves_per_simd;
     float* out = nullptr;
     if (hipMalloc(&out, sizeof(float) * blocks * threads) != hipSuccess) return TX_ERR_NOMEM;
     hipEvent_t e0, e1;
@@ -71,3 +73,5 @@ extern "C" int tx_util_valu_probe(double* lane_ops_per_s)
     *lane_ops_per_s = ops / (ms * 1e-3);
     return TX_OK;
 }
+
+extern "C" int tx_util_valu_probe(double* lane_ops_per_s) { return tx_util_valu_probe_waves(8, lane_ops_per_s); }
