@@ -172,10 +172,17 @@ def test_cli_is_built():
     (["compress", "-f", "bc1", "notpng.png"], 101, "Is this really a PNG file?"),
     (["decompress", "missing.dds"], 101, "Failed to open file"),
     (["decompress", "notdds.dds"], 101, "DDS"),
+    (["decompress", "huge.dds"], 101, "out of range"),
 ])
 def test_cli_arguments(tmp_path, args, code, text):
     (tmp_path / "notpng.png").write_bytes(b"not a png at all")
     (tmp_path / "notdds.dds").write_bytes(b"DDS " + bytes(40))
+    # a legacy DXT1 header claiming 2^32-1 x 2^32-1 pixels (refused before
+    # anything is allocated or sent to the GPU)
+    huge = bytearray(128)
+    struct.pack_into("<4s7I", huge, 0, b"DDS ", 124, 0x1007, 0xFFFFFFFF, 0xFFFFFFFF, 0, 0, 0)
+    struct.pack_into("<2I4s", huge, 4 + 72, 32, 0x4, b"DXT1")
+    (tmp_path / "huge.dds").write_bytes(bytes(huge) + bytes(64))
     r = run_cli(args, tmp_path)
     assert r.returncode == code, r.stderr
     assert text in r.stdout + r.stderr

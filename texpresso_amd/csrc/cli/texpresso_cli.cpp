@@ -21,6 +21,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <new>
 #include <string>
 #include <vector>
 
@@ -146,6 +147,15 @@ std::string default_output(const std::string& infile, const char* ext)
     const size_t dot = name.rfind('.');
     if (dot != std::string::npos && dot != 0) name.resize(dot);
     return name + "." + ext;
+}
+
+// 4 * w * h bytes of RGBA, refusing sizes whose product overflows or that no
+// texture the library accepts can have (sides below 2^31).
+size_t rgba_bytes(size_t w, size_t h)
+{
+    if (w == 0 || h == 0 || w > 0x7FFFFFFFu || h > 0x7FFFFFFFu || w > (SIZE_MAX / 4) / h)
+        fail("Image size out of range: " + std::to_string(w) + "x" + std::to_string(h));
+    return 4 * w * h;
 }
 
 std::string lower(std::string s)
@@ -300,7 +310,7 @@ int run_compress(int argc, char** argv)
     size_t w = 0, h = 0;
     int rc = tx_png_info(file.data(), file.size(), &w, &h);
     if (rc != TX_OK) fail_rc("Failed to read PNG header. Is this really a PNG file?", rc);
-    std::vector<uint8_t> rgba(4 * w * h);
+    std::vector<uint8_t> rgba(rgba_bytes(w, h));
     rc = tx_png_decode(file.data(), file.size(), rgba.data(), rgba.size());
     if (rc != TX_OK) fail_rc("Failed to decode PNG image data", rc);
 
@@ -326,7 +336,7 @@ int run_decompress(int argc, char** argv)
     uint32_t levels = 0;
     int rc = tx_dds_parse_header(file.data(), file.size(), &format, &w, &h, &levels, &off);
     if (rc != TX_OK) fail_rc("Failed to read the DDS file", rc);
-    std::vector<uint8_t> rgba(4 * w * h);
+    std::vector<uint8_t> rgba(rgba_bytes(w, h));
     rc = tx_decompress(format, file.data() + off, file.size() - off, w, h, rgba.data(), rgba.size());
     if (rc != TX_OK) fail_rc("decompress failed", rc);
     std::vector<uint8_t> png(tx_png_encode_bound(w, h));
@@ -340,7 +350,7 @@ int run_decompress(int argc, char** argv)
 
 }  // namespace
 
-int main(int argc, char** argv)
+int run(int argc, char** argv)
 {
     if (argc < 2) {
         fputs(kTopUsage, stderr);
@@ -363,4 +373,13 @@ int main(int argc, char** argv)
         return 0;
     }
     usage_error("unrecognized subcommand '" + cmd + "'", "Usage: texpresso <COMMAND>");
+}
+
+int main(int argc, char** argv)
+{
+    try {
+        return run(argc, argv);
+    } catch (const std::bad_alloc&) {
+        fail("Out of host memory");
+    }
 }
