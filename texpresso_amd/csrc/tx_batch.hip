@@ -211,7 +211,7 @@ struct tx_batch {
     hipGraphExec_t exec = nullptr;
     uint8_t* pipe_mem = nullptr;  // binned-pipeline buffers (tuning.pipeline)
     txp::PipeBufs pipe;
-    bool any_bc1 = false;
+    uint32_t pipe_lists = 0;  // txp::PipeLaunch::lists for the batch's textures
     uint32_t pipe_grid = 0;
     uint64_t pipe_slots = 0;  // record slots covered by the heavy chunks
     uint32_t classify_grid = 0;
@@ -263,7 +263,7 @@ hipError_t launch_batch(tx_batch* b, hipStream_t s)
     p.chunk_blocks = b->tuning.chunk_blocks;
     p.classify_grid = std::min(b->classify_grid, b->nheavy);
     p.search_grid = b->pipe_grid;
-    p.any_bc1 = b->any_bc1;
+    p.lists = b->pipe_lists;
     const hipError_t e = txp::launch_pipeline(p, b->pipe, s);
     if (e != hipSuccess) return e;
     if (nlight && b->light_stream) {
@@ -399,7 +399,7 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
         const uint64_t heavy_blocks = slots;
         b->pipe_slots = slots;
         for (size_t i = 0; i < count; ++i)
-            if (tex[i].variant == 1u || tex[i].variant == 2u) b->any_bc1 = true;
+            if (tex[i].variant < 9u) b->pipe_lists |= txp::lists_for(tex[i].variant / 3u, tex[i].variant % 3u);
         b->pipe_grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * txp::kSearchWavesPerCu,
                                                     std::max<uint64_t>(1, (heavy_blocks + 3) / 4));
         b->classify_grid = (uint32_t)prop.multiProcessorCount * txp::kClassifyWavesPerCu;

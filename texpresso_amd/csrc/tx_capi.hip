@@ -134,7 +134,9 @@ static uint64_t pipeline_threshold(int format, int algorithm)
 {
     const uint64_t t = txc::pipeline_min_blocks();
     const bool bc1 = format == TX_BC1;
-    const uint64_t halves = algorithm == TX_ITERATIVE_CLUSTER_FIT ? (bc1 ? 3 : 2) : (bc1 ? 8 : 4);
+    // measured crossovers: x1 BC2/BC3 IterativeClusterFit, x1.5 BC1 (either
+    // algorithm), x2 BC2/BC3 ClusterFit
+    const uint64_t halves = bc1 ? 3 : (algorithm == TX_ITERATIVE_CLUSTER_FIT ? 2 : 4);
     return t > UINT64_MAX / halves ? UINT64_MAX : t * halves / 2;
 }
 
@@ -186,7 +188,7 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
     p.d_chunks = nullptr;
     p.classify_grid = (uint32_t)std::min<uint64_t>(cwaves, p.nchunks);
     p.search_grid = (uint32_t)std::min<uint64_t>(swaves, (a.nblocks + 3) / 4);
-    p.any_bc1 = a.format == TX_BC1;
+    p.lists = txp::lists_for((uint32_t)a.format, (uint32_t)a.algorithm);
     const hipError_t e = txp::launch_pipeline(p, bufs, s, marks);
     if (hipEventRecord(ctx->pipe_done, s) != hipSuccess) return hipErrorUnknown;
     return e;

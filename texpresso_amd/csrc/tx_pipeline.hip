@@ -14,7 +14,7 @@ namespace txp {
 size_t PipeBufs::bytes_for(uint64_t cap)
 {
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return up(sizeof(ClusterRec) * cap) + 3 * up(4 * cap) + up(sizeof(PipeCtl));
+    return up(sizeof(ClusterRec) * cap) + (1 + kLists) * up(4 * cap) + up(sizeof(PipeCtl));
 }
 
 void PipeBufs::carve(uint8_t* base, uint64_t cap)
@@ -25,10 +25,10 @@ void PipeBufs::carve(uint8_t* base, uint64_t cap)
     off += up(sizeof(ClusterRec) * cap);
     keys = reinterpret_cast<uint32_t*>(base + off);
     off += up(4 * cap);
-    list_a = reinterpret_cast<uint32_t*>(base + off);
-    off += up(4 * cap);
-    list_b = reinterpret_cast<uint32_t*>(base + off);
-    off += up(4 * cap);
+    for (int k = 0; k < kLists; ++k) {
+        list[k] = reinterpret_cast<uint32_t*>(base + off);
+        off += up(4 * cap);
+    }
     ctl = reinterpret_cast<PipeCtl*>(base + off);
     capacity = cap;
 }
@@ -41,7 +41,11 @@ using txk::BatchTex;
 using txp::ClusterRec;
 using txp::PipeCtl;
 
-constexpr uint32_t kKeyA = 1u << 5, kKeyB = 1u << 6;
+using txp::kLists;
+using txp::kListA;
+using txp::kListB;
+using txp::kListC;
+constexpr uint32_t key_bit(int list) { return 1u << (5 + list); }
 
 __global__ void k_pipe_reset(PipeCtl* ctl)
 {
@@ -56,7 +60,7 @@ __device__ __forceinline__ const BatchTex& tex_at(const BatchTex* tex, const Bat
 
 // Per-workgroup bin counts of the classify pass, flushed once at its end.
 struct ClassifyHist {
-    uint32_t a[17], b[17];
+    uint32_t n[kLists][17];
 };
 
 // ClusterFit block on one lane: its record into slot r, its key, and its
@@ -85,9 +89,19 @@ __device__ __forceinline__ uint32_t write_record_lane(ClusterRec* __restrict__ r
     tail[2] = make_uint4(__float_as_uint(vz), __float_as_uint(FLT_MAX), 0u, 0u);
     tail[3] = make_uint4(__float_as_uint(fp.wr), __float_as_uint(fp.wg), __float_as_uint(fp.wb), tex_idx);
     tail[4] = make_uint4(block, info, 0u, 0u);
-    if (FMT == BC1) atomicAdd(&H.a[S.n], 1u);
-    if (need4) atomicAdd(&H.b[S.n], 1u);
-    return (uint32_t)S.n | (FMT == BC1 ? kKeyA : 0u) | (need4 ? kKeyB : 0u);
+    // BC1 ClusterFit: list C (both searches in one pass); otherwise the
+    // 3-colour search for BC1 (list A) and the 4-colour one unless
+    // transparent (list B)
+    uint32_t key = (uint32_t)S.n;
+    if (FMT == BC1 && ALG == CLUSTER_FIT) {
+        key |= key_bit(kListC);
+    } else {
+        if (FMT == BC1) key |= key_bit(kListA);
+        if (need4) key |= key_bit(kListB);
+    }
+    for (int k = 0; k < kLists; ++k)
+        if (key & key_bit(k)) atomicAdd(&H.n[k][S.n], 1u);
+    return key;
 }
 
 // Blocks [first, first + count) of one texture, one per lane, 64 at a time.
@@ -137,10 +151,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ ClassifyHist H;
     const int lane = (int)threadIdx.x;
     fill_lane_tables(LT, lane);
-    if (lane < 17) {
-        H.a[lane] = 0u;
-        H.b[lane] = 0u;
-    }
+    if (lane < 17)
+        for (int k = 0; k < kLists; ++k) H.n[k][lane] = 0u;
     group_sync();
     // chunks hold similar work, so a plain grid-stride loop balances them
     // (a shared work counter would serialise thousands of atomics at launch)
@@ -168,9 +180,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // one flush of the workgroup's bin counts
     group_sync();
     if (lane < 17) {
-        const uint32_t a = H.a[lane], b = H.b[lane];
-        if (a) atomicAdd(&ctl->hist_a[lane], a);
-        if (b) atomicAdd(&ctl->hist_b[lane], b);
+        for (int k = 0; k < kLists; ++k) {
+            const uint32_t c = H.n[k][lane];
+            if (c) atomicAdd(&ctl->hist[k][lane], c);
+        }
     }
 }
 
@@ -183,41 +196,37 @@ constexpr uint32_t kBinSeg = 4096;
 
 __global__ __launch_bounds__(256) void k_pipe_bin(const uint32_t* __restrict__ keys, uint64_t nslots,
                                                   PipeCtl* __restrict__ ctl, uint32_t* __restrict__ list_a,
-                                                  uint32_t* __restrict__ list_b)
+                                                  uint32_t* __restrict__ list_b, uint32_t* __restrict__ list_c)
 {
-    __shared__ uint32_t cnt_a[17], cnt_b[17], base_a[17], base_b[17];
+    __shared__ uint32_t cnt[kLists][17], base[kLists][17];
+    uint32_t* const lists[kLists] = {list_a, list_b, list_c};
     const uint32_t t = threadIdx.x;
-    if (t < 17) {
-        cnt_a[t] = 0u;
-        cnt_b[t] = 0u;
-    }
+    if (t < 17)
+        for (int k = 0; k < kLists; ++k) cnt[k][t] = 0u;
     __syncthreads();
     const uint64_t s0 = (uint64_t)blockIdx.x * kBinSeg;
     const uint64_t s1 = s0 + kBinSeg < nslots ? s0 + kBinSeg : nslots;
     for (uint64_t r = s0 + t; r < s1; r += blockDim.x) {
-        const uint32_t k = keys[r];
-        if (k & kKeyA) atomicAdd(&cnt_a[k & 31u], 1u);
-        if (k & kKeyB) atomicAdd(&cnt_b[k & 31u], 1u);
+        const uint32_t key = keys[r];
+        for (int k = 0; k < kLists; ++k)
+            if (key & key_bit(k)) atomicAdd(&cnt[k][key & 31u], 1u);
     }
     __syncthreads();
     if (t < 17) {
         // bins are laid out 16 first: bin n starts after the counts of bins n+1..16
-        uint32_t oa = 0, ob = 0;
-        for (int m = 16; m > (int)t; --m) {
-            oa += ctl->hist_a[m];
-            ob += ctl->hist_b[m];
+        for (int k = 0; k < kLists; ++k) {
+            uint32_t o = 0;
+            for (int m = 16; m > (int)t; --m) o += ctl->hist[k][m];
+            base[k][t] = cnt[k][t] ? o + atomicAdd(&ctl->pos[k][t], cnt[k][t]) : 0u;
+            cnt[k][t] = 0u;
         }
-        base_a[t] = cnt_a[t] ? oa + atomicAdd(&ctl->pos_a[t], cnt_a[t]) : 0u;
-        base_b[t] = cnt_b[t] ? ob + atomicAdd(&ctl->pos_b[t], cnt_b[t]) : 0u;
-        cnt_a[t] = 0u;
-        cnt_b[t] = 0u;
     }
     __syncthreads();
     for (uint64_t r = s0 + t; r < s1; r += blockDim.x) {
-        const uint32_t k = keys[r];
-        const uint32_t n = k & 31u;
-        if (k & kKeyA) list_a[base_a[n] + atomicAdd(&cnt_a[n], 1u)] = (uint32_t)r;
-        if (k & kKeyB) list_b[base_b[n] + atomicAdd(&cnt_b[n], 1u)] = (uint32_t)r;
+        const uint32_t key = keys[r];
+        const uint32_t n = key & 31u;
+        for (int k = 0; k < kLists; ++k)
+            if (key & key_bit(k)) lists[k][base[k][n] + atomicAdd(&cnt[k][n], 1u)] = (uint32_t)r;
     }
 }
 
@@ -391,6 +400,70 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
 }
 
+// BC1 ClusterFit records (colourfit.rs:44-60 with a single ordering per
+// search, cluster.rs:152-417): the four groups of a wave pull a record each
+// and run compress3 and then, unless the block is transparent, compress4 on
+// it side by side, exactly as the direct kernel does.  With one ordering per
+// search there is nothing to gain from letting the groups run independently,
+// and one pull serves both searches.  Records are sorted by colour count, so
+// the four candidate loops have the same length.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_cluster_bc1(
+    const BatchTex* __restrict__ tex, BatchTex tex0, const ClusterRec* __restrict__ recs,
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor)
+{
+    __shared__ GroupLds lds[4];
+    const int lane = (int)threadIdx.x, g = lane >> 4, l = lane & 15;
+    GroupLds& L = lds[g];
+    uint32_t total = 0;
+    for (int m = 0; m < 17; ++m) total += hist[m];
+    // the wave takes four list positions per step (one per group) from a
+    // private run of kPull; the first run is fixed by the workgroup index,
+    // refills come from the shared cursor, offset past those runs
+    const uint32_t kPull = 4u * min(4u, max(1u, total / (4u * gridDim.x)));
+    uint32_t pool = blockIdx.x * kPull, pool_end = pool + kPull;
+    const uint32_t cursor_base = gridDim.x * kPull;
+    for (;;) {
+        if (pool == pool_end) {
+            uint32_t fresh = 0;
+            if (lane == 0) fresh = atomicAdd(cursor, kPull);
+            pool = __builtin_amdgcn_readfirstlane(fresh) + cursor_base;
+            pool_end = pool + kPull;
+        }
+        const uint32_t p = pool + (uint32_t)g;
+        pool += 4;
+        const bool has = p < total;
+        if (!wave_any(has)) break;
+        ColourSet cs{0, -1, false};
+        FitParams fp{0.0f, 0.0f, 0.0f, 0u};
+        float axx = 0.0f, axy = 0.0f, axz = 0.0f;
+        uint32_t tex_idx = 0, block = 0;
+        if (has) {
+            const ClusterRec& R = recs[list[p]];
+            L.pts[l] = R.pts[l];
+            const uint32_t info = R.info;
+            cs.n = (int)(info & 31u);
+            cs.transparent = ((info >> 8) & 1u) != 0u;
+            const uint32_t byte = (R.remap[l >> 2] >> (8 * (l & 3))) & 255u;
+            cs.remap = byte == 255u ? -1 : (int)byte;
+            fp = FitParams{R.w[0], R.w[1], R.w[2], 0u};
+            axx = R.axis[0];
+            axy = R.axis[1];
+            axz = R.axis[2];
+            tex_idx = R.tex;
+            block = R.block;
+        }
+        group_sync();
+        float best = FLT_MAX;
+        uint32_t lo = 0, hi = 0;  // all-zero block if no candidate wins (colourfit.rs:44-60)
+        run_cluster<3, 1>(L, cs, l, has, axx, axy, axz, fp, best, lo, hi);
+        run_cluster<4, 1>(L, cs, l, has && !cs.transparent, axx, axy, axz, fp, best, lo, hi);
+        if (has && l == 0) {
+            const BatchTex& T = tex_at(tex, tex0, tex_idx);
+            *reinterpret_cast<uint2*>(T.out + (size_t)block * 8u) = make_uint2(lo, hi);
+        }
+    }
+}
+
 }  // namespace txd
 
 namespace txp {
@@ -410,15 +483,19 @@ hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s
                        p.nchunks, p.chunk_blocks, b.recs, b.keys, b.ctl);
     if (mark(1) != hipSuccess) return hipErrorUnknown;
     const uint32_t bin_grid = (uint32_t)((p.nslots + txd::kBinSeg - 1) / txd::kBinSeg);
-    hipLaunchKernelGGL(txd::k_pipe_bin, dim3(bin_grid), dim3(256), 0, s, b.keys, p.nslots, b.ctl, b.list_a,
-                       b.list_b);
+    hipLaunchKernelGGL(txd::k_pipe_bin, dim3(bin_grid), dim3(256), 0, s, b.keys, p.nslots, b.ctl, b.list[kListA],
+                       b.list[kListB], b.list[kListC]);
     if (mark(2) != hipSuccess) return hipErrorUnknown;
-    if (p.any_bc1)
+    if (p.lists & (1u << kListC))
+        hipLaunchKernelGGL(txd::k_pipe_cluster_bc1, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
+                           b.list[kListC], b.ctl->hist[kListC], &b.ctl->cursor[kListC]);
+    if (p.lists & (1u << kListA))
         hipLaunchKernelGGL(txd::k_pipe_search<3>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
-                           b.list_a, b.ctl->hist_a, &b.ctl->cursor_a);
+                           b.list[kListA], b.ctl->hist[kListA], &b.ctl->cursor[kListA]);
     if (mark(3) != hipSuccess) return hipErrorUnknown;
-    hipLaunchKernelGGL(txd::k_pipe_search<4>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs, b.list_b,
-                       b.ctl->hist_b, &b.ctl->cursor_b);
+    if (p.lists & (1u << kListB))
+        hipLaunchKernelGGL(txd::k_pipe_search<4>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
+                           b.list[kListB], b.ctl->hist[kListB], &b.ctl->cursor[kListB]);
     if (mark(4) != hipSuccess) return hipErrorUnknown;
     return hipGetLastError();
 }

@@ -6,13 +6,17 @@
 //   lane; single-colour and range-fit blocks are finished here, ClusterFit
 //   blocks become a ClusterRec and are counted by distinct-colour count n.
 // bin: counting sort of the records by n (16 first) into one list per pass
-//   (list A: BC1 records for the 3-colour search; list B: records that need
-//   the 4-colour search).
+//   (list A: BC1 IterativeClusterFit records for the 3-colour search; list
+//   B: records that need the 4-colour search -- BC1 IterativeClusterFit
+//   without transparency, BC2/BC3 ClusterFit and IterativeClusterFit; list
+//   C: BC1 ClusterFit records, which get both searches in one pass).
 // search<NC>: persistent waves; each 16-lane group pulls the next record
 //   from its list and runs that block's ordering iterations on its own, so
 //   blocks with one ordering and blocks with several share a wave without
 //   waiting for each other, and neighbouring records have the same n (the
 //   same number of candidate rounds).
+// cluster_bc1: BC1 ClusterFit (a single ordering per search): each group
+//   pulls a record and runs the 3-colour then the 4-colour search on it.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -48,17 +52,18 @@ static_assert(offsetof(ClusterRec, remap) == 256 && offsetof(ClusterRec, alpha) 
 // Counters of one pipeline run (cleared by the reset kernel).  Every update
 // is aggregated (per workgroup or per batch of records) so that no counter
 // takes one global atomic per block.
+enum { kListA = 0, kListB = 1, kListC = 2, kLists = 3 };
+
 struct PipeCtl {
-    uint32_t cursor_a, cursor_b;
-    uint32_t hist_a[17], hist_b[17];
-    uint32_t pos_a[17], pos_b[17];
+    uint32_t cursor[kLists];
+    uint32_t hist[kLists][17];  // records per colour count
+    uint32_t pos[kLists][17];   // records placed per colour count (bin pass)
 };
 
 struct PipeBufs {
-    ClusterRec* recs = nullptr;  // capacity records; slot = the block's position among the chunks
-    uint32_t* keys = nullptr;    // capacity: n | in list A << 5 | in list B << 6; 0 = no record
-    uint32_t* list_a = nullptr;  // capacity
-    uint32_t* list_b = nullptr;  // capacity
+    ClusterRec* recs = nullptr;          // capacity records; slot = the block's position among the chunks
+    uint32_t* keys = nullptr;            // capacity: n | in list L << (5 + L); 0 = no record
+    uint32_t* list[kLists] = {nullptr};  // capacity each: record slots sorted by n, 16 first
     PipeCtl* ctl = nullptr;
     uint64_t capacity = 0;
 
@@ -85,13 +90,24 @@ struct PipeLaunch {
     uint64_t nslots;             // record slots the chunks cover (<= capacity)
     uint32_t classify_grid;      // persistent 64-thread workgroups of classify
     uint32_t search_grid;        // persistent 64-thread workgroups of each search
-    bool any_bc1;                // run the 3-colour search
+    uint32_t lists;              // bit L set: records may go to list L (its search is launched)
 };
+
+// The lists a texture's records can go to: BC1 ClusterFit -> C, BC1
+// IterativeClusterFit -> A and B, BC2/BC3 ClusterFit or IterativeClusterFit
+// -> B; RangeFit -> none (classify finishes those blocks).
+inline uint32_t lists_for(uint32_t format, uint32_t algorithm)
+{
+    if (algorithm == 0 || format > 2) return 0u;
+    if (format == 0) return algorithm == 1 ? 1u << kListC : (1u << kListA) | (1u << kListB);
+    return 1u << kListB;
+}
 
 // Launches reset, classify, bin and the searches on `s`.  With `marks`
 // (5 events), marks[0] is recorded before classify, marks[1] after it,
-// marks[2] after bin, marks[3] after the 3-colour search (recorded even when
-// there is none) and marks[4] after the 4-colour search.
+// marks[2] after bin, marks[3] after the BC1 ClusterFit pass and the
+// 3-colour search (recorded even when there are none) and marks[4] after
+// the 4-colour search.
 hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s, hipEvent_t* marks = nullptr);
 
 }  // namespace txp

@@ -198,12 +198,12 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
     // chunks small enough that every resident wave gets work even when the
     // launch holds a single small job (latency), capped by the tuned size
     uint64_t colour_blocks = 0;  // BC1-BC3 (LDS encoder or pipeline); BC4/BC5 go to the alpha-only kernel
-    bool any_bc1_cluster = false;
+    uint32_t lists = 0;  // pipeline lists the launch's records can go to
     for (uint64_t id : ids) {
         const Job& j = jobs[id];
         if (j.spec.format <= TX_BC3)
             colour_blocks += (uint64_t)((j.spec.width + 3) / 4) * ((j.spec.height + 3) / 4);
-        any_bc1_cluster |= j.spec.format == TX_BC1 && j.spec.params.algorithm != TX_RANGE_FIT;
+        lists |= txp::lists_for((uint32_t)j.spec.format, (uint32_t)j.spec.params.algorithm);
     }
     const uint64_t waves = (uint64_t)grid * cfg.tuning.waves_per_group;
     uint64_t want = (colour_blocks + waves - 1) / std::max<uint64_t>(1, waves);
@@ -304,7 +304,7 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
         p.chunk_blocks = chunk;
         p.classify_grid = std::min<uint32_t>(cus * txp::kClassifyWavesPerCu, nheavy);
         p.search_grid = (uint32_t)std::min<uint64_t>((uint64_t)cus * txp::kSearchWavesPerCu, (slot + 3) / 4);
-        p.any_bc1 = any_bc1_cluster;
+        p.lists = lists;
         if (ok && txp::launch_pipeline(p, L.pipe, L.stream) != hipSuccess) ok = false;
     } else if (ok && c &&
                txk::launch_batch_raw(L.d_tex, L.d_chunks, nheavy, nlight, L.d_counter, launch_grid, light_grid,
