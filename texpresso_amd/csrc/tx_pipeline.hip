@@ -89,11 +89,11 @@ __device__ __forceinline__ uint32_t write_record_lane(ClusterRec* __restrict__ r
     tail[2] = make_uint4(__float_as_uint(vz), __float_as_uint(FLT_MAX), 0u, 0u);
     tail[3] = make_uint4(__float_as_uint(fp.wr), __float_as_uint(fp.wg), __float_as_uint(fp.wb), tex_idx);
     tail[4] = make_uint4(block, info, 0u, 0u);
-    // BC1 ClusterFit: list C (both searches in one pass); otherwise the
-    // 3-colour search for BC1 (list A) and the 4-colour one unless
-    // transparent (list B)
+    // ClusterFit: list C (all of the block's searches in one pass);
+    // IterativeClusterFit: the 3-colour search for BC1 (list A) and the
+    // 4-colour one unless transparent (list B)
     uint32_t key = (uint32_t)S.n;
-    if (FMT == BC1 && ALG == CLUSTER_FIT) {
+    if (ALG == CLUSTER_FIT) {
         key |= key_bit(kListC);
     } else {
         if (FMT == BC1) key |= key_bit(kListA);
@@ -400,14 +400,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
 }
 
-// BC1 ClusterFit records (colourfit.rs:44-60 with a single ordering per
-// search, cluster.rs:152-417): the four groups of a wave pull a record each
-// and run compress3 and then, unless the block is transparent, compress4 on
-// it side by side, exactly as the direct kernel does.  With one ordering per
-// search there is nothing to gain from letting the groups run independently,
-// and one pull serves both searches.  Records are sorted by colour count, so
-// the four candidate loops have the same length.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_cluster_bc1(
+// ClusterFit records (colourfit.rs:44-60 with a single ordering per search,
+// cluster.rs:152-417): the four groups of a wave pull a record each and run
+// the block's searches on it side by side, exactly as the direct kernel
+// does -- compress3 and then, unless the block is transparent, compress4 for
+// BC1; compress4 for BC2/BC3.  With one ordering per search there is nothing
+// to gain from letting the groups run independently, and for BC1 one pull
+// serves both searches.  Records are sorted by colour count, so the four
+// candidate loops have the same length.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_cluster(
     const BatchTex* __restrict__ tex, BatchTex tex0, const ClusterRec* __restrict__ recs,
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor)
 {
@@ -436,11 +437,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         ColourSet cs{0, -1, false};
         FitParams fp{0.0f, 0.0f, 0.0f, 0u};
         float axx = 0.0f, axy = 0.0f, axz = 0.0f;
-        uint32_t tex_idx = 0, block = 0;
+        uint32_t tex_idx = 0, block = 0, fmt = BC1, alo = 0, ahi = 0;
         if (has) {
             const ClusterRec& R = recs[list[p]];
             L.pts[l] = R.pts[l];
             const uint32_t info = R.info;
+            fmt = (info >> 12) & 15u;
+            alo = R.alpha[0];
+            ahi = R.alpha[1];
             cs.n = (int)(info & 31u);
             cs.transparent = ((info >> 8) & 1u) != 0u;
             const uint32_t byte = (R.remap[l >> 2] >> (8 * (l & 3))) & 255u;
@@ -455,11 +459,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         group_sync();
         float best = FLT_MAX;
         uint32_t lo = 0, hi = 0;  // all-zero block if no candidate wins (colourfit.rs:44-60)
-        run_cluster<3, 1>(L, cs, l, has, axx, axy, axz, fp, best, lo, hi);
-        run_cluster<4, 1>(L, cs, l, has && !cs.transparent, axx, axy, axz, fp, best, lo, hi);
+        run_cluster<3, 1>(L, cs, l, has && fmt == BC1, axx, axy, axz, fp, best, lo, hi);
+        run_cluster<4, 1>(L, cs, l, has && !(fmt == BC1 && cs.transparent), axx, axy, axz, fp, best, lo, hi);
         if (has && l == 0) {
             const BatchTex& T = tex_at(tex, tex0, tex_idx);
-            *reinterpret_cast<uint2*>(T.out + (size_t)block * 8u) = make_uint2(lo, hi);
+            if (fmt == BC1)
+                *reinterpret_cast<uint2*>(T.out + (size_t)block * 8u) = make_uint2(lo, hi);
+            else
+                *reinterpret_cast<uint4*>(T.out + (size_t)block * 16u) = make_uint4(alo, ahi, lo, hi);
         }
     }
 }
@@ -487,7 +494,7 @@ hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s
                        b.list[kListB], b.list[kListC]);
     if (mark(2) != hipSuccess) return hipErrorUnknown;
     if (p.lists & (1u << kListC))
-        hipLaunchKernelGGL(txd::k_pipe_cluster_bc1, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
+        hipLaunchKernelGGL(txd::k_pipe_cluster, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
                            b.list[kListC], b.ctl->hist[kListC], &b.ctl->cursor[kListC]);
     if (p.lists & (1u << kListA))
         hipLaunchKernelGGL(txd::k_pipe_search<3>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,

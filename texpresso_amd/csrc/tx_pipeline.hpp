@@ -8,15 +8,15 @@
 // bin: counting sort of the records by n (16 first) into one list per pass
 //   (list A: BC1 IterativeClusterFit records for the 3-colour search; list
 //   B: records that need the 4-colour search -- BC1 IterativeClusterFit
-//   without transparency, BC2/BC3 ClusterFit and IterativeClusterFit; list
-//   C: BC1 ClusterFit records, which get both searches in one pass).
+//   without transparency, BC2/BC3 IterativeClusterFit; list C: ClusterFit
+//   records of every format, which get all their searches in one pass).
 // search<NC>: persistent waves; each 16-lane group pulls the next record
 //   from its list and runs that block's ordering iterations on its own, so
 //   blocks with one ordering and blocks with several share a wave without
 //   waiting for each other, and neighbouring records have the same n (the
 //   same number of candidate rounds).
-// cluster_bc1: BC1 ClusterFit (a single ordering per search): each group
-//   pulls a record and runs the 3-colour then the 4-colour search on it.
+// cluster: ClusterFit (a single ordering per search): each group pulls a
+//   record and runs its searches on it (BC1: 3-colour then 4-colour).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -93,21 +93,21 @@ struct PipeLaunch {
     uint32_t lists;              // bit L set: records may go to list L (its search is launched)
 };
 
-// The lists a texture's records can go to: BC1 ClusterFit -> C, BC1
-// IterativeClusterFit -> A and B, BC2/BC3 ClusterFit or IterativeClusterFit
-// -> B; RangeFit -> none (classify finishes those blocks).
+// The lists a texture's records can go to: ClusterFit -> C; BC1
+// IterativeClusterFit -> A and B; BC2/BC3 IterativeClusterFit -> B; RangeFit
+// -> none (classify finishes those blocks).
 inline uint32_t lists_for(uint32_t format, uint32_t algorithm)
 {
     if (algorithm == 0 || format > 2) return 0u;
-    if (format == 0) return algorithm == 1 ? 1u << kListC : (1u << kListA) | (1u << kListB);
-    return 1u << kListB;
+    if (algorithm == 1) return 1u << kListC;
+    return format == 0 ? (1u << kListA) | (1u << kListB) : 1u << kListB;
 }
 
 // Launches reset, classify, bin and the searches on `s`.  With `marks`
 // (5 events), marks[0] is recorded before classify, marks[1] after it,
-// marks[2] after bin, marks[3] after the BC1 ClusterFit pass and the
-// 3-colour search (recorded even when there are none) and marks[4] after
-// the 4-colour search.
+// marks[2] after bin, marks[3] after the ClusterFit pass and the 3-colour
+// search (recorded even when there are none) and marks[4] after the
+// 4-colour search.
 hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s, hipEvent_t* marks = nullptr);
 
 }  // namespace txp
