@@ -416,8 +416,10 @@ def run_suite(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    # the GPU's clocks settle over the first ~30 launches (1.33 -> 1.11 ms per
+    # launch of the split search on the default workload); time the steady state
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
