@@ -126,7 +126,7 @@ def _download(bufs):
     return [d_out.download(n) for _, d_out, n in bufs]
 
 
-def suite_cfg0(tx, texgen, O, threads, steps):
+def suite_cfg0(tx, texgen, O, threads, steps, warmup):
     """BC1 ClusterFit on the seeded 512x512 texture."""
     W = H = 512
     img = texgen.rgb_texture(W, H, texgen.SEED_BASE)
@@ -134,7 +134,7 @@ def suite_cfg0(tx, texgen, O, threads, steps):
     d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
     n = tx.Format.Bc1.compressed_size(W, H)
     d_out = tx.DeviceBuffer(n)
-    ms, _ = time_device(tx, tx.Format.Bc1, d_in, W, H, p, d_out, n, steps, 2)
+    ms, _ = time_device(tx, tx.Format.Bc1, d_in, W, H, p, d_out, n, steps, warmup)
     got = d_out.download()
     op = O.params(1, p.weights, False)
     t0 = time.perf_counter()
@@ -148,7 +148,7 @@ def suite_cfg0(tx, texgen, O, threads, steps):
             "valu_tflops": round(flops / (ms / 1e3) / 1e12, 2)}
 
 
-def suite_cfg2(tx, texgen, O, threads, steps, tuning_path):
+def suite_cfg2(tx, texgen, O, threads, steps, warmup, tuning_path):
     """configs[2]: full mip chain of a seeded 4096x4096 texture, per format;
     the chain is one batch with the autotuned tuning for that format."""
     seed = texgen.SEED_BASE + 2
@@ -165,9 +165,10 @@ def suite_cfg2(tx, texgen, O, threads, steps, tuning_path):
         bufs, descs = _upload_set(tx, items)
         best, _ = tx.tuned("cfg2-%s-%s" % (name, alg.name), descs, table_path=tuning_path, retune=True)
         batch = tx.Batch(descs, best)
-        batch.run_timed(1)
+        batch.run_timed(max(1, warmup))
         ms_batch = batch.run_timed(steps) / steps
         got = _download(bufs)
+        tx.time_per_texture(descs, 2)
         ms_plain = tx.time_per_texture(descs, steps) / steps
         op = O.params(int(alg), p.weights, False)
         t0 = time.perf_counter()
@@ -212,19 +213,20 @@ def cfg3_items(tx, texgen, count=1000):
     return items
 
 
-def suite_cfg3(tx, texgen, O, threads, steps, tuning_path, count=1000):
+def suite_cfg3(tx, texgen, O, threads, steps, warmup, tuning_path, count=1000):
     items = cfg3_items(tx, texgen, count)
     px = sum(im.shape[0] * im.shape[1] for _, _, im in items)
     bufs, descs = _upload_set(tx, items)
     sig = "cfg3-%d" % count
     best, tune_ms = tx.tuned(sig, descs, table_path=tuning_path, retune=True)
     batch = tx.Batch(descs, best)
-    batch.run_timed(1)
+    batch.run_timed(max(1, warmup))
     ms_batch = batch.run_timed(steps) / steps
     got = _download(bufs)
     default = tx.Batch(descs)
-    default.run_timed(1)
+    default.run_timed(max(1, warmup))
     ms_default = default.run_timed(steps) / steps
+    tx.time_per_texture(descs, 2)
     ms_plain = tx.time_per_texture(descs, steps) / steps
     plain = _download(bufs)
     mismatch_vs_plain = sum(int(np.count_nonzero(a != b)) for a, b in zip(got, plain))
@@ -395,14 +397,14 @@ def run_suite(args):
     which = args.suite.split(",")
     out = []
     if "cfg0" in which:
-        out.append(suite_cfg0(tx, texgen, O, threads, args.steps))
+        out.append(suite_cfg0(tx, texgen, O, threads, args.steps, args.warmup))
         print(json.dumps(out[-1]), flush=True)
     if "cfg2" in which:
-        for r in suite_cfg2(tx, texgen, O, threads, args.steps, args.tuning_table):
+        for r in suite_cfg2(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table):
             out.append(r)
             print(json.dumps(r), flush=True)
     if "cfg3" in which:
-        out.append(suite_cfg3(tx, texgen, O, threads, args.steps, args.tuning_table, args.cfg3_count))
+        out.append(suite_cfg3(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table, args.cfg3_count))
         print(json.dumps(out[-1]), flush=True)
     if "cfg4" in which:
         for r in suite_cfg4(tx, texgen, [float(x) for x in args.qps.split(",")]):
