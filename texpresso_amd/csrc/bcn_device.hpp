@@ -206,21 +206,21 @@ __device__ __forceinline__ void fix_range(int& mn, int& mx, int steps)
     if (mx - mn < steps) mn = max(mx - steps, 0);
 }
 
-// alpha.rs:79-118 for one pixel: first strict minimum over the 8 codes
+// alpha.rs:79-118 for one pixel: the first strict minimum of (v - code)^2
+// over the 8 codes, and that squared distance.  The square is monotone in
+// |v - code|, so the same code wins the smallest key 8|v - code| + j: the
+// distance orders the keys and, at equal distance, the lower index.  Values
+// and codes are below 256, so v_sad_u16 forms each key from 8v and 8*code in
+// one instruction.
 __device__ __forceinline__ uint32_t fit8(int v, const int (&codes)[8], uint32_t& least)
 {
-    least = 0xFFFFFFFFu;
-    uint32_t index = 0;
+    const uint32_t v8 = (uint32_t)v << 3;
+    uint32_t key = 0xFFFFFFFFu;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int d = v - codes[j];
-        const uint32_t dd = (uint32_t)(d * d);
-        if (dd < least) {
-            least = dd;
-            index = (uint32_t)j;
-        }
-    }
-    return index;
+    for (int j = 0; j < 8; ++j) key = min(key, __builtin_amdgcn_sad_u16(v8, (uint32_t)codes[j] << 3, (uint32_t)j));
+    const uint32_t d = (key >> 3) & 255u;  // |v - code| <= 255
+    least = d * d;
+    return key & 7u;
 }
 
 // alpha.rs:187-256 (BC3 alpha, BC4, BC5 channels)
