@@ -492,6 +492,9 @@ def main():
     gpu_out = d_out.download()
     # per-kernel split of the same launches (events between the pipeline's kernels)
     phases = tx.time_phases(fmt, d_in.ptr, W, H, params, d_out.ptr, out_len, args.steps)
+    # job latency: every job (this texture, input resident in HBM) timed on
+    # its own, events around its launches and a wait for it to finish
+    lat_dev = [tx.time_compress(fmt, d_in.ptr, W, H, params, d_out.ptr, out_len, 1) for _ in range(max(50, args.steps))]
 
     # BC5 on the normal map (second half of the workload), same protocol
     n5 = tx.Format.Bc5.compressed_size(W, H)
@@ -527,12 +530,16 @@ def main():
     flat = np.ascontiguousarray(img.reshape(-1))
     fmt.compress(flat, W, H, params, host_out)
     t_host = []
-    for _ in range(5):
+    for _ in range(20):
         t0 = time.perf_counter()
         fmt.compress(flat, W, H, params, host_out)
         t_host.append(time.perf_counter() - t0)
     line["secondary"]["host_buffers_ms"] = round(1e3 * float(np.median(t_host)), 4)
     line["secondary"]["host_buffers_mpix_s"] = round(mpix / float(np.median(t_host)), 1)
+    line["latency"] = {"job": "one %dx%d BC3 IterativeClusterFit texture" % (W, H),
+                       "device_resident": dict(_pcts(lat_dev), max_ms=round(float(max(lat_dev)), 3),
+                                               samples=len(lat_dev)),
+                       "host_buffers_incl_pcie": dict(_pcts([1e3 * t for t in t_host]), samples=len(t_host))}
 
     import ctypes
     probe = ctypes.c_double()
