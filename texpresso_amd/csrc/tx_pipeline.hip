@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // reserves one range per non-empty bin with a single global atomic, then
 // places its records inside those ranges (order within a bin is free: it
 // only steers which records share a wave).
-constexpr uint32_t kBinSeg = 4096;
+constexpr uint32_t kBinSeg = 1024;
 
 __global__ __launch_bounds__(256) void k_pipe_bin(const uint32_t* __restrict__ keys, uint64_t nslots,
                                                   PipeCtl* __restrict__ ctl, uint32_t* __restrict__ list_a,
