@@ -126,18 +126,17 @@ static txk::EncodeArgs encode_args(int format, const tx_params* p, const uint8_t
 }
 
 // Blocks from which a texture goes through the binned pipeline: the set
-// threshold for BC2/BC3 IterativeClusterFit, 1.5x for BC1
-// IterativeClusterFit, 2x for BC2/BC3 ClusterFit and 4x for BC1 ClusterFit
-// (measured crossovers: BC1's two search passes and single-ordering blocks
-// gain less from independent per-block scheduling).
+// threshold for BC2/BC3 IterativeClusterFit, 1.25x for BC2/BC3 ClusterFit,
+// 1.5x for BC1 ClusterFit and 3x for BC1 IterativeClusterFit (measured
+// crossovers against the direct kernel, 256^2-1448^2: BC1's two search
+// passes and single-ordering blocks gain less from independent per-block
+// scheduling).
 static uint64_t pipeline_threshold(int format, int algorithm)
 {
     const uint64_t t = txc::pipeline_min_blocks();
-    const bool bc1 = format == TX_BC1;
-    // measured crossovers: x1 BC2/BC3 IterativeClusterFit, x1.5 BC1 (either
-    // algorithm), x2 BC2/BC3 ClusterFit
-    const uint64_t halves = bc1 ? 3 : (algorithm == TX_ITERATIVE_CLUSTER_FIT ? 2 : 4);
-    return t > UINT64_MAX / halves ? UINT64_MAX : t * halves / 2;
+    const bool bc1 = format == TX_BC1, iterative = algorithm == TX_ITERATIVE_CLUSTER_FIT;
+    const uint64_t quarters = bc1 ? (iterative ? 12 : 6) : (iterative ? 4 : 5);
+    return t > UINT64_MAX / quarters ? UINT64_MAX : t * quarters / 4;
 }
 
 // Encodes one texture on `s`: ClusterFit textures of at least
