@@ -98,7 +98,11 @@ int tx_decompress_block(int format, const uint8_t* block, size_t block_len, uint
 /* Device-resident variants of Format::compress / Format::decompress: all
  * pointers are device memory (4-byte aligned), `stream` is a hipStream_t
  * (NULL = the library's stream for the current device).  The call enqueues
- * the kernel and returns without synchronising. */
+ * the kernel and returns without synchronising.  Textures that take the
+ * binned ClusterFit pipeline (tx_set_pipeline_threshold) share one
+ * intermediate buffer per device; the library makes each such call's stream
+ * wait for the previous one's kernels, so these calls run one after another
+ * even when issued on different streams. */
 int tx_compress_device(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
                        uint8_t* d_output, size_t out_len, void* stream);
 int tx_decompress_device(int format, const uint8_t* d_data, size_t width, size_t height, uint8_t* d_output,
