@@ -126,8 +126,8 @@ def _download(bufs):
     return [d_out.download(n) for _, d_out, n in bufs]
 
 
-def suite_cfg0(tx, texgen, O, threads, steps, warmup):
-    """BC1 ClusterFit on the seeded 512x512 texture."""
+def suite_bc1_512(tx, texgen, O, threads, steps, warmup):
+    """BC1 ClusterFit on the seeded 512x512 texture (the CPU-runnable case)."""
     W = H = 512
     img = texgen.rgb_texture(W, H, texgen.SEED_BASE)
     p = tx.Params()
@@ -142,14 +142,14 @@ def suite_cfg0(tx, texgen, O, threads, steps, warmup):
     t_cpu = time.perf_counter() - t0
     bad = int(np.count_nonzero(np.any(got.reshape(-1, 8) != want.reshape(-1, 8), axis=1)))
     flops = algorithmic_flops(stats)
-    return {"config": "cfg0 BC1 ClusterFit 512x512", "gpu_mpix_s": round(W * H / 1e6 / (ms / 1e3), 1),
+    return {"config": "bc1-512: BC1 ClusterFit 512x512", "gpu_mpix_s": round(W * H / 1e6 / (ms / 1e3), 1),
             "ms": round(ms, 4), "cpu_mpix_s": round(W * H / 1e6 / t_cpu, 3), "cpu_threads": threads,
             "mismatched_blocks": bad, "blocks": W * H // 16,
             "valu_tflops": round(flops / (ms / 1e3) / 1e12, 2)}
 
 
-def suite_cfg2(tx, texgen, O, threads, steps, warmup, tuning_path):
-    """configs[2]: full mip chain of a seeded 4096x4096 texture, per format;
+def suite_chains(tx, texgen, O, threads, steps, warmup, tuning_path):
+    """Full mip chain of a seeded 4096x4096 texture, per format;
     the chain is one batch with the autotuned tuning for that format."""
     seed = texgen.SEED_BASE + 2
     base = texgen.with_alpha_field(texgen.rgb_texture(4096, 4096, seed), seed)
@@ -163,7 +163,7 @@ def suite_cfg2(tx, texgen, O, threads, steps, warmup, tuning_path):
         p = tx.Params(alg)
         items = [(fmt, p, l) for l in chain]
         bufs, descs = _upload_set(tx, items)
-        best, _ = tx.tuned("cfg2-%s-%s" % (name, alg.name), descs, table_path=tuning_path, retune=True)
+        best, _ = tx.tuned("chain4096-%s-%s" % (name, alg.name), descs, table_path=tuning_path, retune=True)
         batch = tx.Batch(descs, best)
         batch.run_timed(max(1, warmup))
         ms_batch = batch.run_timed(steps) / steps
@@ -181,7 +181,7 @@ def suite_cfg2(tx, texgen, O, threads, steps, warmup, tuning_path):
         bad = sum(int(np.count_nonzero(g != w)) for g, w in zip(got, want))
         bs = tx.Format(fmt).block_size()
         traffic = 4 * px + sum(len(w) for w in want)
-        rows.append({"config": "cfg2 %s %s 4096 chain (%d levels)" % (name, alg.name, len(chain)),
+        rows.append({"config": "chains: %s %s 4096 chain (%d levels)" % (name, alg.name, len(chain)),
                      "mpix": round(px / 1e6, 3), "batch_ms": round(ms_batch, 4),
                      "batch_mpix_s": round(px / 1e6 / (ms_batch / 1e3), 1), "per_level_ms": round(ms_plain, 4),
                      "per_level_mpix_s": round(px / 1e6 / (ms_plain / 1e3), 1),
@@ -197,8 +197,8 @@ def suite_cfg2(tx, texgen, O, threads, steps, warmup, tuning_path):
     return rows
 
 
-def cfg3_items(tx, texgen, count=1000):
-    """configs[3]: `count` textures, sides log-uniform in [3, 2048], full mip
+def batch_items(tx, texgen, count=1000):
+    """`count` textures, sides log-uniform in [3, 2048], full mip
     chains, formats uniform over BC1/BC3/BC4/BC5, Params::default()."""
     rng = np.random.Generator(np.random.PCG64(texgen.SEED_BASE + 3))
     ws = texgen.log_uniform_sides(rng, count, 3, 2048)
@@ -213,11 +213,11 @@ def cfg3_items(tx, texgen, count=1000):
     return items
 
 
-def suite_cfg3(tx, texgen, O, threads, steps, warmup, tuning_path, count=1000):
-    items = cfg3_items(tx, texgen, count)
+def suite_batch(tx, texgen, O, threads, steps, warmup, tuning_path, count=1000):
+    items = batch_items(tx, texgen, count)
     px = sum(im.shape[0] * im.shape[1] for _, _, im in items)
     bufs, descs = _upload_set(tx, items)
-    sig = "cfg3-%d" % count
+    sig = "batch-%d" % count
     best, tune_ms = tx.tuned(sig, descs, table_path=tuning_path, retune=True)
     batch = tx.Batch(descs, best)
     batch.run_timed(max(1, warmup))
@@ -239,7 +239,7 @@ def suite_cfg3(tx, texgen, O, threads, steps, warmup, tuning_path, count=1000):
         want = O.compress(fmt, im, im.shape[1], im.shape[0], O.params(int(p.algorithm), p.weights, False),
                           threads=threads)
         bad += int(np.count_nonzero(got[i] != want))
-    return {"config": "cfg3 %d textures + mip chains (%d levels), BC1/BC3/BC4/BC5 default params" % (count, len(items)),
+    return {"config": "batch: %d textures + mip chains (%d levels), BC1/BC3/BC4/BC5 default params" % (count, len(items)),
             "mpix": round(px / 1e6, 2), "batch_ms": round(ms_batch, 3),
             "batch_mpix_s": round(px / 1e6 / (ms_batch / 1e3), 1), "batch_default_tuning_ms": round(ms_default, 3),
             "per_texture_ms": round(ms_plain, 3), "per_texture_mpix_s": round(px / 1e6 / (ms_plain / 1e3), 1),
@@ -248,8 +248,8 @@ def suite_cfg3(tx, texgen, O, threads, steps, warmup, tuning_path, count=1000):
             "oracle_sample_textures": int(len(sample)), "oracle_sample_mismatched_bytes": bad}
 
 
-def cfg4_pool(tx, texgen, count=24):
-    """configs[4] job inputs: seeded textures, side log-uniform in [64, 4096],
+def serving_pool(tx, texgen, count=24):
+    """Serving job inputs: seeded textures, side log-uniform in [64, 4096],
     format uniform over BC1/BC3/BC5, IterativeClusterFit."""
     rng = np.random.Generator(np.random.PCG64(texgen.SEED_BASE + 4))
     sides = texgen.log_uniform_sides(rng, count, 64, 4096)
@@ -357,8 +357,8 @@ def sequential_run(tx, pool, arrivals, picks, params):
     return lat, (last - first) / 1e9
 
 
-def suite_cfg4(tx, texgen, rates, pool_size=24, seconds=2.0):
-    pool = cfg4_pool(tx, texgen, pool_size)
+def suite_serving(tx, texgen, rates, pool_size=24, seconds=2.0):
+    pool = serving_pool(tx, texgen, pool_size)
     params = tx.Params(tx.Algorithm.IterativeClusterFit)
     # untimed warm-up: every pool texture once through a server and once
     # sequentially (kernel loading, scratch growth)
@@ -377,7 +377,7 @@ def suite_cfg4(tx, texgen, rates, pool_size=24, seconds=2.0):
             tx, pool, arrivals, picks, params,
             dict(hbm_budget=16 << 30, max_inflight=3, max_jobs_per_launch=64))
         lat_seq, wall_seq = sequential_run(tx, pool, arrivals, picks, params)
-        row = {"config": "cfg4 serving, Poisson %g jobs/s, %d jobs, BC1/BC3/BC5 IterativeClusterFit, sides 64-4096" %
+        row = {"config": "serving: Poisson %g jobs/s, %d jobs, BC1/BC3/BC5 IterativeClusterFit, sides 64-4096" %
                          (rate, n),
                "offered_mpix_s": round(mpix / arrivals[-1], 1),
                "server_mpix_s": round(mpix / wall, 1), "server_device_latency": _pcts(lat_dev),
@@ -396,18 +396,18 @@ def run_suite(args):
     threads = cpu_threads()
     which = args.suite.split(",")
     out = []
-    if "cfg0" in which:
-        out.append(suite_cfg0(tx, texgen, O, threads, args.steps, args.warmup))
+    if "bc1-512" in which:
+        out.append(suite_bc1_512(tx, texgen, O, threads, args.steps, args.warmup))
         print(json.dumps(out[-1]), flush=True)
-    if "cfg2" in which:
-        for r in suite_cfg2(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table):
+    if "chains" in which:
+        for r in suite_chains(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table):
             out.append(r)
             print(json.dumps(r), flush=True)
-    if "cfg3" in which:
-        out.append(suite_cfg3(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table, args.cfg3_count))
+    if "batch" in which:
+        out.append(suite_batch(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table, args.batch_count))
         print(json.dumps(out[-1]), flush=True)
-    if "cfg4" in which:
-        for r in suite_cfg4(tx, texgen, [float(x) for x in args.qps.split(",")]):
+    if "serving" in which:
+        for r in suite_serving(tx, texgen, [float(x) for x in args.qps.split(",")]):
             out.append(r)
             print(json.dumps(r), flush=True)
     if args.suite_out:
@@ -429,12 +429,12 @@ def main():
                     help="print the plain FP32 VALU issue rate at 1..8 waves per SIMD and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
-    ap.add_argument("--suite", default="", help="comma list of cfg0,cfg2,cfg3: per-config lines instead of the "
+    ap.add_argument("--suite", default="", help="comma list of bc1-512,chains,batch,serving: per-workload lines instead of the "
                                                  "default line")
     ap.add_argument("--suite-out", default="", help="write the suite results to this JSON file")
     ap.add_argument("--tuning-table", default=os.path.join(ROOT, "texpresso_amd", "tuning.json"))
-    ap.add_argument("--cfg3-count", type=int, default=1000)
-    ap.add_argument("--qps", default="100,250,500,1000,2000", help="cfg4 arrival-rate sweep (jobs/s)")
+    ap.add_argument("--batch-count", type=int, default=1000)
+    ap.add_argument("--qps", default="100,250,500,1000,2000", help="serving arrival-rate sweep (jobs/s)")
     ap.add_argument("--pipeline-threshold", type=int, default=-1,
                     help="blocks from which one texture goes through the binned ClusterFit pipeline "
                          "(-1: library default; 0: always; a huge value: never)")
