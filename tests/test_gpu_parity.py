@@ -115,7 +115,7 @@ def test_adversarial_blocks(gpu, oracle, fmt, alg, wi, alpha):
     assert np.array_equal(got, want), report(fmt, got.reshape(-1), want.reshape(-1), oracle)
 
 
-def test_config0_bc1_clusterfit_512(gpu, oracle):
+def test_bc1_clusterfit_512_texture(gpu, oracle):
     # BC1 ClusterFit on the seeded 512x512 texture, compared byte for byte
     img = texgen.rgb_texture(512, 512, texgen.SEED_BASE)
     p = tx.Params()
