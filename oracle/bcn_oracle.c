@@ -28,10 +28,15 @@
  *     candidate search of ClusterFit: float64 closed forms (weighted
  *     least squares, eigenvectors, exhaustive candidate minimum) within a
  *     stated tolerance, and stable-sort order of the projections.
+ *   - IterativeClusterFit's second iteration: the second ordering is the
+ *     stable sort along end - start of the first ordering's best split
+ *     (quantised, binary32), and a first-iteration winner means the second
+ *     ordering repeated or did not improve -- float64 replay of the first
+ *     search; a flipped or unquantised axis fails it.
  *   - parity unpinned: the bit-level operation order of ClusterFit/RangeFit
- *     for n > 3 distinct colours, perceptual weights, iterative refinement
- *     and alpha-weighted colour sets has no reference vector; it is pinned
- *     only by faithful restatement plus the float64 checks above.
+ *     for n > 3 distinct colours, perceptual weights, iterations from the
+ *     third on and alpha-weighted colour sets has no reference vector; it is
+ *     pinned only by faithful restatement plus the float64 checks above.
  */
 #include <float.h>
 #include <math.h>

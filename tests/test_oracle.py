@@ -255,6 +255,92 @@ def test_ordering_is_stable_sort_of_projections():
         assert list(fit.order[:n]) == want
 
 
+def _split_errors4(P, W, cw, grid):
+    """Every 4-colour split (i, j, k) of ordered points P (weights W), in
+    loop order: float64 least-squares endpoints from the 2x2 normal
+    equations, quantised, and the split's error; None where singular."""
+    n = len(W)
+    out = []
+    for i in range(n):
+        for j in range(i, n + 1):
+            for k in range(max(j, 1), n + 1):
+                al, be = _split_weights(n, i, j, k, True)
+                a2, b2, ab = (W * al * al).sum(), (W * be * be).sum(), (W * al * be).sum()
+                det = a2 * b2 - ab * ab
+                if det <= 1e-9 * max(a2 * b2, 1e-30):
+                    out.append(((i, j, k), None, None, None, None))
+                    continue
+                ax = (W[:, None] * al[:, None] * P).sum(0)
+                bx = (W[:, None] * be[:, None] * P).sum(0)
+                a = (ax * b2 - bx * ab) / det
+                b = (bx * a2 - ax * ab) / det
+                out.append(((i, j, k), a, b, _err(P, W, al, be, _quant(a, grid), _quant(b, grid), cw), (a, b)))
+    return out
+
+
+def _near_grid_boundary(v, grid, tol=1e-3):
+    f = np.clip(v, 0, 1) * grid + 0.5
+    return np.any(np.abs(f - np.round(f)) < tol)
+
+
+def _quant32(v, grid):
+    """The encoder's snap in binary32: trunc(grid * v + 0.5) * (1 / grid)."""
+    v32 = np.clip(np.float32(v), np.float32(0), np.float32(1)).astype(np.float32)
+    g = grid.astype(np.float32)
+    return (np.trunc(g * v32 + np.float32(0.5)) * (np.float32(1) / g)).astype(np.float32)
+
+
+def _stable_order(P32, axis32):
+    keys = (P32[:, 0] * axis32[0] + P32[:, 1] * axis32[1]) + P32[:, 2] * axis32[2]
+    keys = np.where(np.isfinite(keys), keys, np.float32(np.inf))
+    return sorted(range(len(keys)), key=lambda q: (keys[q], q))
+
+
+def test_iterative_cluster_fit_second_ordering():
+    """IterativeClusterFit (cluster.rs:222-249): the second ordering sorts the
+    points along end - start of the first ordering's best split, quantised,
+    in binary32.  A block whose winner comes from iteration 1 must have been
+    searched on exactly that ordering; a block whose winner is iteration 0
+    must have a second ordering that repeats the first or does not beat its
+    error.  The first search is redone in float64 and blocks whose best split
+    or grid snap is within rounding of another are skipped."""
+    rng = np.random.default_rng(9)
+    grid = np.array([31.0, 63.0, 31.0])
+    cw = np.array(O.PERCEPTUAL)
+    seen = {0: 0, 1: 0}
+    for px in _random_blocks(rng, 700, ncol=6):
+        t, _ = _trace(O.BC3, px, alg=O.ITERATIVE_CLUSTER_FIT)
+        fit = t.fit4
+        n = t.count
+        if not fit.won or n < 3 or fit.iteration > 1:
+            continue
+        P32 = np.array([[t.points[q][c] for c in range(3)] for q in range(n)], np.float32)
+        W = np.array(t.weights[:n], np.float64)
+        order0 = _stable_order(P32, np.array(t.principle[:], np.float32))
+        errs = [e for e in _split_errors4(P32[order0].astype(np.float64), W[order0], cw, grid) if e[1] is not None]
+        errs.sort(key=lambda e: e[3])
+        if len(errs) < 2 or errs[1][3] - errs[0][3] < 1e-4:
+            continue
+        (_, a, b, best0, _) = errs[0]
+        if _near_grid_boundary(a, grid) or _near_grid_boundary(b, grid):
+            continue
+        axis1 = _quant32(b, grid) - _quant32(a, grid)
+        order1 = _stable_order(P32, axis1)
+        if fit.iteration == 1:
+            assert list(fit.order[:n]) == order1
+            seen[1] += 1
+        elif order1 == order0:
+            seen[0] += 1  # the repeated ordering ends the search
+        else:
+            errs1 = [e[3] for e in _split_errors4(P32[order1].astype(np.float64), W[order1], cw, grid)
+                     if e[1] is not None]
+            if abs(min(errs1) - best0) < 1e-4:
+                continue
+            assert min(errs1) > best0
+            seen[0] += 1
+    assert seen[1] > 20 and seen[0] > 20, seen
+
+
 # ------------------------------------------------------------ alpha encoder
 
 def test_bc4_constant_blocks_round_trip():
