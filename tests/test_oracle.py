@@ -127,6 +127,36 @@ def test_colour_set_and_covariance_closed_form():
         assert np.allclose(np.array(t.cov[:]), want, rtol=1e-4, atol=1e-6)
 
 
+def test_colour_set_alpha_weighted_closed_form():
+    """weigh_colour_by_alpha (colourset.rs:35-112): a point's weight is the sum
+    of (alpha + 1) / 256 over its pixels (exact in binary32: multiples of
+    1/256 below 17), then sqrt, which rounded from float64 is the correctly
+    rounded binary32 root; the covariance follows with those weights."""
+    rng = np.random.default_rng(11)
+    for px in _random_blocks(rng, 300, ncol=6):
+        px[:, 3] = rng.integers(0, 256, 16)
+        t, _ = _trace(O.BC3, px, alpha=True)
+        cols, sums = [], []
+        for p in px:
+            key = tuple(int(v) for v in p[:3])
+            a = (int(p[3]) + 1) / 256.0
+            if key in cols:
+                sums[cols.index(key)] += a
+            else:
+                cols.append(key)
+                sums.append(a)
+        pts, sums = np.array(cols, np.float64) / 255.0, np.array(sums)
+        n = len(sums)
+        assert t.count == n
+        w = np.sqrt(sums)
+        assert np.array_equal(np.array(t.weights[:n], np.float32), w.astype(np.float32))
+        c = (pts * w[:, None]).sum(0) / w.sum()
+        d = pts - c
+        cov = (d[:, :, None] * d[:, None, :] * w[:, None, None]).sum(0)
+        want = np.array([cov[0, 0], cov[0, 1], cov[0, 2], cov[1, 1], cov[1, 2], cov[2, 2]])
+        assert np.allclose(np.array(t.cov[:]), want, rtol=1e-4, atol=1e-6)
+
+
 def test_power_iteration_closed_form():
     # eight steps of v <- C v / max(C v) from (1,1,1), in float64
     rng = np.random.default_rng(2)
