@@ -34,12 +34,12 @@
  *     ordering repeated or did not improve -- float64 replay of the first
  *     search; a flipped or unquantised axis fails it.
  *   - alpha-weighted colour sets: point weights sqrt(sum (alpha + 1) / 256)
- *     bit-exact and their covariance, float64 closed form.
+ *     bit-exact, their covariance and their first-iteration least-squares
+ *     endpoints and error, float64 closed forms.
  *   - parity unpinned: the bit-level operation order of ClusterFit/RangeFit
- *     for n > 3 distinct colours, perceptual weights, iterations from the
- *     third on and the split searches over alpha-weighted sets has no
- *     reference vector; it is pinned only by faithful restatement plus the
- *     float64 checks above.
+ *     for n > 3 distinct colours, perceptual weights and iterations from the
+ *     third on has no reference vector; it is pinned only by faithful
+ *     restatement plus the float64 checks above.
  */
 #include <float.h>
 #include <math.h>

@@ -212,19 +212,38 @@ def _err(pts, w, al, be, a, b, cw):
     return float(((r * r * w[:, None]).sum(0) * cw).sum() - ((pts * pts * w[:, None]).sum(0) * cw).sum())
 
 
+def _alpha_weights_of(px):
+    """Distinct colours in first-occurrence order and their summed
+    (alpha + 1) / 256 weights."""
+    cols, sums = [], []
+    for p in px:
+        key = tuple(int(v) for v in p[:3])
+        a = (int(p[3]) + 1) / 256.0
+        if key in cols:
+            sums[cols.index(key)] += a
+        else:
+            cols.append(key)
+            sums.append(a)
+    return np.array(cols, np.float64) / 255.0, np.array(sums, np.float64)
+
+
+@pytest.mark.parametrize("alpha", [False, True], ids=["counts", "alphaw"])
 @pytest.mark.parametrize("four", [True, False])
-def test_cluster_fit_least_squares_and_error(four):
-    rng = np.random.default_rng(3 if four else 4)
+def test_cluster_fit_least_squares_and_error(four, alpha):
+    rng = np.random.default_rng((3 if four else 4) + (10 if alpha else 0))
     grid = np.array([31.0, 63.0, 31.0])
     cw = np.array(O.PERCEPTUAL)
     checked = 0
     for px in _random_blocks(rng, 400, ncol=rng.integers(3, 9)):
         fmt = O.BC3 if four else O.BC1
-        t, _ = _trace(fmt, px)
+        if alpha:
+            # BC1 excludes alpha < 128 from the colour set; stay above it there
+            px[:, 3] = rng.integers(0 if four else 128, 256, 16)
+        t, _ = _trace(fmt, px, alpha=alpha)
         fit = t.fit4 if four else t.fit3
         if not fit.won or fit.iteration != 0:
             continue
-        pts, cnt = _weights_of(px)
+        pts, cnt = _alpha_weights_of(px) if alpha else _weights_of(px)
         n = len(cnt)
         w = np.sqrt(cnt)
         order = list(fit.order[:n])
