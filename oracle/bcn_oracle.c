@@ -28,17 +28,17 @@
  *     candidate search of ClusterFit: float64 closed forms (weighted
  *     least squares, eigenvectors, exhaustive candidate minimum) within a
  *     stated tolerance, and stable-sort order of the projections.
- *   - IterativeClusterFit's second iteration: the second ordering is the
- *     stable sort along end - start of the first ordering's best split
+ *   - IterativeClusterFit's orderings: each ordering after the first is the
+ *     stable sort along end - start of the previous ordering's best split
  *     (quantised, binary32), and a first-iteration winner means the second
- *     ordering repeated or did not improve -- float64 replay of the first
- *     search; a flipped or unquantised axis fails it.
+ *     ordering repeated or did not improve -- float64 replay of the searches
+ *     up to the third ordering; a flipped or unquantised axis fails it.
  *   - alpha-weighted colour sets: point weights sqrt(sum (alpha + 1) / 256)
  *     bit-exact, their covariance and their first-iteration least-squares
  *     endpoints and error, float64 closed forms.
  *   - parity unpinned: the bit-level operation order of ClusterFit/RangeFit
  *     for n > 3 distinct colours, perceptual weights and iterations from the
- *     third on has no reference vector; it is pinned only by faithful
+ *     fourth on has no reference vector; it is pinned only by faithful
  *     restatement plus the float64 checks above.
  */
 #include <float.h>

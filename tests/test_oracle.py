@@ -345,49 +345,71 @@ def _stable_order(P32, axis32):
     return sorted(range(len(keys)), key=lambda q: (keys[q], q))
 
 
-def test_iterative_cluster_fit_second_ordering():
-    """IterativeClusterFit (cluster.rs:222-249): the second ordering sorts the
-    points along end - start of the first ordering's best split, quantised,
-    in binary32.  A block whose winner comes from iteration 1 must have been
-    searched on exactly that ordering; a block whose winner is iteration 0
-    must have a second ordering that repeats the first or does not beat its
-    error.  The first search is redone in float64 and blocks whose best split
-    or grid snap is within rounding of another are skipped."""
+def _best_split_unambiguous(P32, W, order, cw, grid):
+    """float64 replay of one search: the best split's float64 endpoints, its
+    error and the next ordering's axis (binary32 snap of end - start), or
+    None when the best split or a grid snap is within rounding of another."""
+    errs = [e for e in _split_errors4(P32[order].astype(np.float64), W[order], cw, grid) if e[1] is not None]
+    errs.sort(key=lambda e: e[3])
+    if len(errs) < 2 or errs[1][3] - errs[0][3] < 1e-4:
+        return None
+    (_, a, b, best, _) = errs[0]
+    if _near_grid_boundary(a, grid) or _near_grid_boundary(b, grid):
+        return None
+    return best, _quant32(b, grid) - _quant32(a, grid)
+
+
+def test_iterative_cluster_fit_orderings():
+    """IterativeClusterFit (cluster.rs:222-249): each ordering after the first
+    sorts the points along end - start of the previous ordering's best split,
+    quantised, in binary32.  The searches are replayed in float64 up to the
+    winning iteration, whose ordering must be the replayed one; a block whose
+    winner is iteration 0 must have a second ordering that repeats the first
+    or does not beat its error.  Blocks where a best split or grid snap is
+    within rounding of another are skipped."""
     rng = np.random.default_rng(9)
     grid = np.array([31.0, 63.0, 31.0])
     cw = np.array(O.PERCEPTUAL)
-    seen = {0: 0, 1: 0}
-    for px in _random_blocks(rng, 700, ncol=6):
+    seen = {0: 0, 1: 0, 2: 0}
+    for px in _random_blocks(rng, 1500, ncol=6):
         t, _ = _trace(O.BC3, px, alg=O.ITERATIVE_CLUSTER_FIT)
         fit = t.fit4
         n = t.count
-        if not fit.won or n < 3 or fit.iteration > 1:
+        if not fit.won or n < 3 or fit.iteration > 2:
             continue
         P32 = np.array([[t.points[q][c] for c in range(3)] for q in range(n)], np.float32)
         W = np.array(t.weights[:n], np.float64)
-        order0 = _stable_order(P32, np.array(t.principle[:], np.float32))
-        errs = [e for e in _split_errors4(P32[order0].astype(np.float64), W[order0], cw, grid) if e[1] is not None]
-        errs.sort(key=lambda e: e[3])
-        if len(errs) < 2 or errs[1][3] - errs[0][3] < 1e-4:
+        order = _stable_order(P32, np.array(t.principle[:], np.float32))
+        step = _best_split_unambiguous(P32, W, order, cw, grid)
+        if step is None:
             continue
-        (_, a, b, best0, _) = errs[0]
-        if _near_grid_boundary(a, grid) or _near_grid_boundary(b, grid):
+        best0, axis = step
+        if fit.iteration == 0:
+            nxt = _stable_order(P32, axis)
+            if nxt != order:
+                errs1 = [e[3] for e in _split_errors4(P32[nxt].astype(np.float64), W[nxt], cw, grid)
+                         if e[1] is not None]
+                if abs(min(errs1) - best0) < 1e-4:
+                    continue
+                assert min(errs1) > best0
+            seen[0] += 1  # a repeated ordering ends the search
             continue
-        axis1 = _quant32(b, grid) - _quant32(a, grid)
-        order1 = _stable_order(P32, axis1)
-        if fit.iteration == 1:
-            assert list(fit.order[:n]) == order1
-            seen[1] += 1
-        elif order1 == order0:
-            seen[0] += 1  # the repeated ordering ends the search
-        else:
-            errs1 = [e[3] for e in _split_errors4(P32[order1].astype(np.float64), W[order1], cw, grid)
-                     if e[1] is not None]
-            if abs(min(errs1) - best0) < 1e-4:
-                continue
-            assert min(errs1) > best0
-            seen[0] += 1
-    assert seen[1] > 20 and seen[0] > 20, seen
+        ambiguous = False
+        for it in range(1, fit.iteration + 1):
+            order = _stable_order(P32, axis)
+            if it == fit.iteration:
+                break
+            step = _best_split_unambiguous(P32, W, order, cw, grid)
+            if step is None:
+                ambiguous = True
+                break
+            axis = step[1]
+        if ambiguous:
+            continue
+        assert list(fit.order[:n]) == order, fit.iteration
+        seen[fit.iteration] += 1
+    assert seen[0] > 20 and seen[1] > 20 and seen[2] > 5, seen
+    print("orderings pinned per winning iteration:", seen)
 
 
 # ------------------------------------------------------------ alpha encoder
