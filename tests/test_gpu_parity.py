@@ -115,6 +115,37 @@ def test_adversarial_blocks(gpu, oracle, fmt, alg, wi, alpha):
     assert np.array_equal(got, want), report(fmt, got.reshape(-1), want.reshape(-1), oracle)
 
 
+def _random_weights(seed):
+    """Seeded colour-weight triples: unit-sum, tiny, large, one channel zero."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(3):
+        w = rng.random(3) + 0.05
+        out.append(tuple(float(x) for x in w / w.sum()))
+    out.append(tuple(float(x) for x in rng.random(3) * 1e-3 + 1e-4))
+    out.append(tuple(float(x) for x in rng.random(3) * 40.0 + 1.0))
+    w = rng.random(3)
+    w[int(rng.integers(0, 3))] = 0.0
+    out.append(tuple(float(x) for x in w))
+    return out
+
+
+@pytest.mark.parametrize("fmt", [0, 2], ids=["bc1", "bc3"])
+@pytest.mark.parametrize("alg", [tx.Algorithm.ClusterFit, tx.Algorithm.IterativeClusterFit], ids=lambda a: a.name)
+@pytest.mark.parametrize("alpha", [False, True], ids=["plain", "alphaw"])
+def test_random_weight_sets(gpu, oracle, fmt, alg, alpha):
+    # Params::weights is a free float triple: the fits multiply by it in
+    # their error sums, so non-uniform, very small, very large and zero
+    # weights each take other rounding paths through the same expressions
+    blocks, masks = make_blocks(2000 + 7 * fmt + int(alg) + (50 if alpha else 0), per_class=48)
+    for wi, weights in enumerate(_random_weights(31 + fmt + 3 * int(alg) + (1 if alpha else 0))):
+        p = _params(alg, weights, alpha)
+        print("weights:", fmt, alg.name, alpha, wi, weights, flush=True)
+        got = tx.compress_blocks(fmt, blocks, masks, p)
+        want = oracle.compress_blocks(fmt, blocks, masks, _oparams(oracle, p))
+        assert np.array_equal(got, want), (weights, report(fmt, got.reshape(-1), want.reshape(-1), oracle))
+
+
 def test_bc1_clusterfit_512_texture(gpu, oracle):
     # BC1 ClusterFit on the seeded 512x512 texture, compared byte for byte
     img = texgen.rgb_texture(512, 512, texgen.SEED_BASE)
