@@ -125,3 +125,21 @@ def test_pipeline_device_entry_repeats(pipeline_always, oracle):
         d_out.upload(np.full(n, 0x5A, np.uint8))
         tx.Format.Bc3.compress_device(d_in.ptr, w, h, p, d_out.ptr, n)
         assert np.array_equal(d_out.download(), want)
+
+
+@pytest.mark.parametrize("fmt", [0, 2], ids=["bc1", "bc3"])
+@pytest.mark.parametrize("alg", ALGS, ids=lambda a: a.name)
+@pytest.mark.parametrize("alpha", [False, True], ids=["plain", "alphaw"])
+def test_pipeline_random_weights(pipeline_always, oracle, fmt, alg, alpha):
+    # the records carry Params::weights to the searches: seeded non-uniform,
+    # tiny, large and one-zero-channel triples through the binned pipeline
+    from tests.test_gpu_parity import _random_weights
+    w = h = 256
+    img = texgen.with_alpha_field(texgen.rgb_texture(w, h, 600 + fmt + int(alg)), 601 + fmt)
+    for weights in _random_weights(70 + fmt + 2 * int(alg) + (1 if alpha else 0)):
+        p = tx.Params(alg, weights, alpha)
+        print("pipeline weights:", fmt, alg.name, alpha, weights, flush=True)
+        got = encode(fmt, img, w, h, p)
+        want = oracle.compress(fmt, img, w, h, oracle.params(int(alg), p.weights, alpha),
+                               threads=oracle.default_threads())
+        assert np.array_equal(got, want), weights
