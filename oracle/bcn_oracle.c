@@ -22,8 +22,23 @@
  *   - single-colour lookup tables: regenerated here and compared byte for
  *     byte with the reference tables (tests/golden/single_lut.bin) and their
  *     SHA-256.
- *   - the BC3 alpha encoder: exhaustive brute force over constant, two-value
- *     and fix_range cases (integer only).
+ *   - the BC3 alpha encoder: every constant block and every two-value block
+ *     (all 32,896 pairs, three layouts) decodes exactly; hand-derived vectors
+ *     for fix_range at both edges of 0..255 and for blocks decided by the
+ *     7-point codebook's slots 0-1 holding min5/max5 (alpha.rs:238-239).
+ *   - SingleColourFit: every grey and 1,500 random colours predicted from the
+ *     reference's tables (index 0 unless source 1 is strictly better; BC1's
+ *     4-colour fit only when strictly better), decoded pixels exactly the
+ *     tables' error away.
+ *   - RangeFit: float64 principal axis, first extreme points, integer grid
+ *     snap, nearest code under the colour weights and 3-/4-colour choice on
+ *     4,500 blocks of 2-16 colours under perceptual, uniform and skewed
+ *     weights, masks and BC1 transparency (blocks within a rounding margin
+ *     of a tie skipped: 257 of them).
+ *   - planted mistakes: tests/test_oracle_pins.py builds copies of this file
+ *     with one plausible mistake each (squared range weights, tie rules of
+ *     the single-colour and alpha choices, min7/max7 in the 7-point slots,
+ *     fix_range's upper clamp) and requires the pins above to fail on them.
  *   - covariance, power iteration, least-squares endpoints, error formula and
  *     candidate search of ClusterFit: float64 closed forms (weighted
  *     least squares, eigenvectors, exhaustive candidate minimum) within a
@@ -36,10 +51,12 @@
  *   - alpha-weighted colour sets: point weights sqrt(sum (alpha + 1) / 256)
  *     bit-exact, their covariance and their first-iteration least-squares
  *     endpoints and error, float64 closed forms.
- *   - parity unpinned: the bit-level operation order of ClusterFit/RangeFit
- *     for n > 3 distinct colours, perceptual weights and iterations from the
- *     fourth on has no reference vector; it is pinned only by faithful
- *     restatement plus the float64 checks above.
+ *   - parity unpinned at the bit level: the binary32 operation order of
+ *     ClusterFit/RangeFit for n > 3 distinct colours, perceptual weights and
+ *     iterations from the fourth on has no reference vector; the float64
+ *     checks above pin every decision that is not within rounding of a tie,
+ *     and near-ties (including the first-extreme rule of the range scan on
+ *     equal projections) rest on faithful restatement.
  */
 #include <float.h>
 #include <math.h>

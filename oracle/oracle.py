@@ -3,6 +3,7 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, never by the texpresso_amd package.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -47,25 +48,45 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
+def load(path):
+    """Loads a build of the restatement from `path` with its argument types set."""
+    L = ctypes.CDLL(path)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    L.ora_compress_block_masked.argtypes = [ctypes.c_int, u8p, ctypes.c_uint32, ctypes.POINTER(OraParams), u8p,
+                                            ctypes.POINTER(OraBlockStats)]
+    L.ora_compress.argtypes = [ctypes.c_int, u8p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                               ctypes.POINTER(OraParams), u8p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    L.ora_compress_blocks.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.POINTER(OraParams), ctypes.c_void_p, ctypes.c_int]
+    L.ora_decompress_block.argtypes = [ctypes.c_int, u8p, u8p]
+    L.ora_decompress.argtypes = [ctypes.c_int, u8p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, u8p,
+                                 ctypes.c_size_t]
+    L.ora_single_lut.argtypes = [u8p]
+    L.ora_block_size.restype = ctypes.c_size_t
+    L.ora_num_blocks.restype = ctypes.c_size_t
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        u8p = ctypes.POINTER(ctypes.c_uint8)
-        L.ora_compress_block_masked.argtypes = [ctypes.c_int, u8p, ctypes.c_uint32, ctypes.POINTER(OraParams), u8p,
-                                                ctypes.POINTER(OraBlockStats)]
-        L.ora_compress.argtypes = [ctypes.c_int, u8p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
-                                   ctypes.POINTER(OraParams), u8p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
-        L.ora_decompress_block.argtypes = [ctypes.c_int, u8p, u8p]
-        L.ora_decompress.argtypes = [ctypes.c_int, u8p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, u8p,
-                                     ctypes.c_size_t]
-        L.ora_single_lut.argtypes = [u8p]
-        L.ora_block_size.restype = ctypes.c_size_t
-        L.ora_num_blocks.restype = ctypes.c_size_t
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def swapped(path):
+    """Runs the enclosed code against another build of the restatement (the
+    planted-mistake checks build copies of bcn_oracle.c with one change)."""
+    global _lib
+    old = lib()
+    _lib = load(path)
+    try:
+        yield
+    finally:
+        _lib = old
 
 
 def params(algorithm=CLUSTER_FIT, weights=PERCEPTUAL, weigh_colour_by_alpha=False):
@@ -121,10 +142,7 @@ def compress_blocks(fmt, blocks, masks, p, threads=None):
     blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
     masks = np.ascontiguousarray(masks, dtype=np.uint32).reshape(-1)
     out = np.zeros((blocks.shape[0], block_size(fmt)), np.uint8)
-    L = lib()
-    L.ora_compress_blocks.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
-                                      ctypes.POINTER(OraParams), ctypes.c_void_p, ctypes.c_int]
-    rc = L.ora_compress_blocks(fmt, blocks.ctypes.data, masks.ctypes.data, blocks.shape[0], ctypes.byref(p),
+    rc = lib().ora_compress_blocks(fmt, blocks.ctypes.data, masks.ctypes.data, blocks.shape[0], ctypes.byref(p),
                                out.ctypes.data, threads or default_threads())
     if rc != 0:
         raise ValueError("oracle compress_blocks failed (rc=%d)" % rc)
