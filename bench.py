@@ -80,12 +80,29 @@ def algorithmic_flops(stats):
     return float(f.sum())
 
 
-def cpu_threads():
+def cpu_counts():
+    """(threads used, CPUs online, CPUs in this process's affinity mask).
+
+    The CPU baseline runs on every CPU this process may use, unless
+    OMP_NUM_THREADS names a smaller share (a host that runs one job per GPU
+    sets it to the cores that belong to each GPU's job)."""
+    online = os.cpu_count() or 1
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        affinity = online
+    n = affinity
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    if share > 0:
+        n = min(n, share)
+    return max(1, n), online, affinity
+
+
+def cpu_threads():
+    return cpu_counts()[0]
 
 
 def cpu_model():
@@ -96,6 +113,21 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def valu_probe(waves_per_simd):
+    """Plain FP32 lane-operations per second measured by the benchmark's
+    probe library (texpresso_amd/libtx_bench_probe.so, built beside the
+    encoder library and not part of it); None when it is absent or fails."""
+    import ctypes
+    path = os.path.join(ROOT, "texpresso_amd", "libtx_bench_probe.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    L.txb_valu_probe.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.txb_valu_probe.restype = ctypes.c_int
+    rate = ctypes.c_double()
+    return rate.value if L.txb_valu_probe(waves_per_simd, ctypes.byref(rate)) == 0 else None
 
 
 def time_device(tx, fmt, d_in, w, h, params, d_out, out_len, steps, warmup):
@@ -126,73 +158,133 @@ def _download(bufs):
     return [d_out.download(n) for _, d_out, n in bufs]
 
 
-def suite_bc1_512(tx, texgen, O, threads, steps, warmup):
-    """BC1 ClusterFit on the seeded 512x512 texture (the CPU-runnable case)."""
-    W = H = 512
-    img = texgen.rgb_texture(W, H, texgen.SEED_BASE)
+def bc1_cluster_gpu(tx, texgen, size, steps, warmup):
+    """BC1 ClusterFit on the seeded size x size RGB texture (alpha 255,
+    default perceptual weights), device-resident: the timed GPU half of a
+    bc1 case (512 is the CPU-runnable configuration)."""
+    img = texgen.rgb_texture(size, size, texgen.SEED_BASE)
     p = tx.Params()
     d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
-    n = tx.Format.Bc1.compressed_size(W, H)
+    n = tx.Format.Bc1.compressed_size(size, size)
     d_out = tx.DeviceBuffer(n)
-    ms, _ = time_device(tx, tx.Format.Bc1, d_in, W, H, p, d_out, n, steps, warmup)
+    ms, _ = time_device(tx, tx.Format.Bc1, d_in, size, size, p, d_out, n, steps, warmup)
     got = d_out.download()
-    op = O.params(1, p.weights, False)
+    d_in.free()
+    d_out.free()
+    return {"size": size, "img": img, "got": got, "ms": ms, "weights": p.weights}
+
+
+def bc1_cluster_cpu(O, g, threads):
+    """The CPU half of a bc1 case: byte parity with the oracle and the
+    oracle's own rate on the host CPUs."""
+    W = H = g["size"]
+    op = O.params(1, g["weights"], False)
     t0 = time.perf_counter()
-    want, stats = O.compress(0, img, W, H, op, threads=threads, stats=True)
+    want, stats = O.compress(0, g["img"], W, H, op, threads=threads, stats=True)
     t_cpu = time.perf_counter() - t0
-    bad = int(np.count_nonzero(np.any(got.reshape(-1, 8) != want.reshape(-1, 8), axis=1)))
+    bad = int(np.count_nonzero(np.any(g["got"].reshape(-1, 8) != want.reshape(-1, 8), axis=1)))
     flops = algorithmic_flops(stats)
-    return {"config": "bc1-512: BC1 ClusterFit 512x512", "gpu_mpix_s": round(W * H / 1e6 / (ms / 1e3), 1),
-            "ms": round(ms, 4), "cpu_mpix_s": round(W * H / 1e6 / t_cpu, 3), "cpu_threads": threads,
-            "mismatched_blocks": bad, "blocks": W * H // 16,
+    ms = g["ms"]
+    gpu = W * H / 1e6 / (ms / 1e3)
+    cpu = W * H / 1e6 / t_cpu
+    return {"config": "bc1-%d: BC1 ClusterFit %dx%d" % (W, W, H), "gpu_mpix_s": round(gpu, 1),
+            "ms": round(ms, 4), "cpu_mpix_s": round(cpu, 3), "cpu_threads": threads,
+            "gpu_over_cpu": round(gpu / cpu, 1), "mismatched_blocks": bad, "blocks": W * H // 16,
+            "input_sha256": hashlib.sha256(g["img"].tobytes()).hexdigest()[:16],
             "valu_tflops": round(flops / (ms / 1e3) / 1e12, 2)}
+
+
+def bc1_cluster_case(tx, texgen, O, size, threads, steps, warmup):
+    return bc1_cluster_cpu(O, bc1_cluster_gpu(tx, texgen, size, steps, warmup), threads)
+
+
+def suite_bc1_512(tx, texgen, O, threads, steps, warmup):
+    return bc1_cluster_case(tx, texgen, O, 512, threads, steps, warmup)
+
+
+def chain_4096(texgen):
+    """Full mip chain (13 levels down to 1x1) of the seeded 4096x4096 texture."""
+    seed = texgen.SEED_BASE + 2
+    base = texgen.with_alpha_field(texgen.rgb_texture(4096, 4096, seed), seed)
+    return texgen.mip_chain(base)
+
+
+def chain_gpu(tx, chain, name, fmt, alg, steps, warmup, tuning_path, retune):
+    """One format/algorithm over the whole chain, device-resident: the chain
+    as one batch (the tuning the autotuner table holds for it) and as one
+    plain launch per level; the timed GPU half of a chain case."""
+    p = tx.Params(alg)
+    items = [(fmt, p, l) for l in chain]
+    bufs, descs = _upload_set(tx, items)
+    best, _ = tx.tuned("chain4096-%s-%s" % (name, alg.name), descs, table_path=tuning_path, retune=retune)
+    batch = tx.Batch(descs, best)
+    batch.run_timed(max(1, warmup))
+    ms_batch = batch.run_timed(steps) / steps
+    got = _download(bufs)
+    batch.close()
+    tx.time_per_texture(descs, max(2, warmup // 4))
+    ms_plain = tx.time_per_texture(descs, steps) / steps
+    got_plain = _download(bufs)
+    for d_in, d_out, _ in bufs:
+        d_in.free()
+        d_out.free()
+    return {"name": name, "fmt": fmt, "alg": alg, "weights": p.weights, "got": got, "got_plain": got_plain,
+            "ms_batch": ms_batch, "ms_plain": ms_plain, "tuning": best.as_dict(),
+            "block_size": tx.Format(fmt).block_size()}
+
+
+def chain_cpu(O, chain, g, threads):
+    """The CPU half of a chain case: every level of both GPU outputs
+    compared with the oracle, which is also timed on the host CPUs over the
+    same levels."""
+    px = sum(l.shape[0] * l.shape[1] for l in chain)
+    op = O.params(int(g["alg"]), g["weights"], False)
+    t0 = time.perf_counter()
+    want, flops = [], 0.0
+    for l in chain:
+        w_, st = O.compress(g["fmt"], l, l.shape[1], l.shape[0], op, threads=threads, stats=True)
+        want.append(w_)
+        flops += algorithmic_flops(st)
+    t_cpu = time.perf_counter() - t0
+    bs = g["block_size"]
+
+    def bad_blocks(outs):
+        return sum(int(np.count_nonzero(np.any(o.reshape(-1, bs) != w.reshape(-1, bs), axis=1)))
+                   for o, w in zip(outs, want))
+
+    traffic = 4 * px + sum(len(w) for w in want)
+    ms_batch, ms_plain = g["ms_batch"], g["ms_plain"]
+    return {"config": "chains: %s %s 4096 chain (%d levels)" % (g["name"], g["alg"].name, len(chain)),
+            "mpix": round(px / 1e6, 3), "blocks": int(sum(len(w) // bs for w in want)),
+            "batch_ms": round(ms_batch, 4), "batch_mpix_s": round(px / 1e6 / (ms_batch / 1e3), 1),
+            "per_level_ms": round(ms_plain, 4), "per_level_mpix_s": round(px / 1e6 / (ms_plain / 1e3), 1),
+            "batch_over_per_level": round(ms_plain / ms_batch, 3),
+            "cpu_mpix_s": round(px / 1e6 / t_cpu, 3), "cpu_threads": threads,
+            "gpu_over_cpu": round((px / ms_batch / 1e3) / (px / t_cpu / 1e6), 1),
+            "mismatched_blocks": bad_blocks(g["got"]), "mismatched_blocks_per_level": bad_blocks(g["got_plain"]),
+            "tuning": g["tuning"],
+            "valu_tflops": round(flops / (ms_batch / 1e3) / 1e12, 3),
+            "valu_frac_of_157": round(flops / (ms_batch / 1e3) / 1e12 / VALU_PEAK_TFLOPS, 4),
+            "hbm_gbs": round(traffic / (ms_batch / 1e3) / 1e9, 1),
+            "hbm_frac": round(traffic / (ms_batch / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def chain_case(tx, O, chain, name, fmt, alg, threads, steps, warmup, tuning_path, retune):
+    return chain_cpu(O, chain, chain_gpu(tx, chain, name, fmt, alg, steps, warmup, tuning_path, retune), threads)
+
+
+CHAIN_CASES = [("BC1", 0, "RangeFit"), ("BC1", 0, "ClusterFit"), ("BC2", 1, "ClusterFit"),
+               ("BC3", 2, "IterativeClusterFit"), ("BC4", 3, "ClusterFit"), ("BC1", 0, "IterativeClusterFit")]
 
 
 def suite_chains(tx, texgen, O, threads, steps, warmup, tuning_path):
     """Full mip chain of a seeded 4096x4096 texture, per format;
     the chain is one batch with the autotuned tuning for that format."""
-    seed = texgen.SEED_BASE + 2
-    base = texgen.with_alpha_field(texgen.rgb_texture(4096, 4096, seed), seed)
-    chain = texgen.mip_chain(base)
-    px = sum(l.shape[0] * l.shape[1] for l in chain)
+    chain = chain_4096(texgen)
     rows = []
-    cases = [("BC1", 0, tx.Algorithm.RangeFit), ("BC1", 0, tx.Algorithm.ClusterFit),
-             ("BC2", 1, tx.Algorithm.ClusterFit), ("BC3", 2, tx.Algorithm.IterativeClusterFit),
-             ("BC4", 3, tx.Algorithm.ClusterFit)]
-    for name, fmt, alg in cases:
-        p = tx.Params(alg)
-        items = [(fmt, p, l) for l in chain]
-        bufs, descs = _upload_set(tx, items)
-        best, _ = tx.tuned("chain4096-%s-%s" % (name, alg.name), descs, table_path=tuning_path, retune=True)
-        batch = tx.Batch(descs, best)
-        batch.run_timed(max(1, warmup))
-        ms_batch = batch.run_timed(steps) / steps
-        got = _download(bufs)
-        tx.time_per_texture(descs, 2)
-        ms_plain = tx.time_per_texture(descs, steps) / steps
-        op = O.params(int(alg), p.weights, False)
-        t0 = time.perf_counter()
-        want, flops = [], 0.0
-        for l in chain:
-            w_, st = O.compress(fmt, l, l.shape[1], l.shape[0], op, threads=threads, stats=True)
-            want.append(w_)
-            flops += algorithmic_flops(st)
-        t_cpu = time.perf_counter() - t0
-        bad = sum(int(np.count_nonzero(g != w)) for g, w in zip(got, want))
-        bs = tx.Format(fmt).block_size()
-        traffic = 4 * px + sum(len(w) for w in want)
-        rows.append({"config": "chains: %s %s 4096 chain (%d levels)" % (name, alg.name, len(chain)),
-                     "mpix": round(px / 1e6, 3), "batch_ms": round(ms_batch, 4),
-                     "batch_mpix_s": round(px / 1e6 / (ms_batch / 1e3), 1), "per_level_ms": round(ms_plain, 4),
-                     "per_level_mpix_s": round(px / 1e6 / (ms_plain / 1e3), 1),
-                     "cpu_mpix_s": round(px / 1e6 / t_cpu, 3), "cpu_threads": threads,
-                     "gpu_over_cpu": round((px / ms_batch / 1e3) / (px / t_cpu / 1e6), 1),
-                     "mismatched_bytes": bad, "block_bytes": bs, "tuning": best.as_dict(),
-                     "valu_tflops": round(flops / (ms_batch / 1e3) / 1e12, 3),
-                     "valu_frac_of_157": round(flops / (ms_batch / 1e3) / 1e12 / VALU_PEAK_TFLOPS, 4),
-                     "hbm_gbs": round(traffic / (ms_batch / 1e3) / 1e9, 1),
-                     "hbm_frac": round(traffic / (ms_batch / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
-        batch.close()
+    for name, fmt, alg in CHAIN_CASES:
+        rows.append(chain_case(tx, O, chain, name, fmt, tx.Algorithm[alg], threads, steps, warmup, tuning_path,
+                               retune=True))
         print("suite: " + json.dumps(rows[-1]), file=sys.stderr, flush=True)
     return rows
 
@@ -448,13 +540,11 @@ def main():
         run_suite(args)
         return
     if args.valu_sweep:
-        import ctypes
-        import texpresso_amd as tx
-        rate = ctypes.c_double()
         for w in range(1, 9):
-            if tx.lib().tx_util_valu_probe_waves(w, ctypes.byref(rate)) != 0:
+            rate = valu_probe(w)
+            if rate is None:
                 sys.exit("valu probe failed")
-            print(json.dumps({"waves_per_simd": w, "plain_fp32_tops": round(rate.value / 1e12, 2)}), flush=True)
+            print(json.dumps({"waves_per_simd": w, "plain_fp32_tops": round(rate / 1e12, 2)}), flush=True)
         return
     if args.copy_calibration:
         import texpresso_amd as tx
@@ -474,6 +564,27 @@ def main():
 
     if tx.lib().tx_device_arch() is None:
         raise SystemExit("no gfx950 device usable by libtexpresso_hip.so")
+
+    # GPU phases first, CPU phases (oracle parity, CPU baselines) last, so the
+    # GPU runs its timed work back to back without idling between them
+    pending = []
+    chain = None
+    if not args.no_cpu:
+        # secondaries: BC1 ClusterFit (the metric's other format) at 512^2 and
+        # 2048^2, and the 4096^2 mip chain in BC1 and BC3
+        # IterativeClusterFit, batched and per level
+        for size in (512, 2048):
+            pending.append(("bc1_clusterfit_%d" % size, "bc1", bc1_cluster_gpu(tx, texgen, size, args.steps,
+                                                                             args.warmup)))
+            print("bench: BC1 ClusterFit %d^2 %.4f ms/step" % (size, pending[-1][2]["ms"]), file=sys.stderr,
+                  flush=True)
+        chain = chain_4096(texgen)
+        for name, f, alg in (("BC1", 0, "IterativeClusterFit"), ("BC3", 2, "IterativeClusterFit")):
+            g = chain_gpu(tx, chain, name, f, tx.Algorithm[alg], args.steps, args.warmup, args.tuning_table,
+                          retune=False)
+            pending.append(("chain4096_%s_%s" % (name.lower(), alg.lower()), "chain", g))
+            print("bench: 4096^2 chain %s %s batched %.3f ms, per level %.3f ms" %
+                  (name, alg, g["ms_batch"], g["ms_plain"]), file=sys.stderr, flush=True)
 
     W = H = args.size
     seed = texgen.SEED_BASE + 1
@@ -541,14 +652,13 @@ def main():
                                                samples=len(lat_dev)),
                        "host_buffers_incl_pcie": dict(_pcts([1e3 * t for t in t_host]), samples=len(t_host))}
 
-    import ctypes
-    probe = ctypes.c_double()
-    if tx.lib().tx_util_valu_probe(ctypes.byref(probe)) == 0:
-        line["secondary"]["valu_plain_fp32_measured_tops"] = round(probe.value / 1e12, 2)
+    probe = valu_probe(8)
+    if probe is not None:
+        line["secondary"]["valu_plain_fp32_measured_tops"] = round(probe / 1e12, 2)
 
     if not args.no_cpu:
         from oracle import oracle as O
-        threads = cpu_threads()
+        threads, online, affinity = cpu_counts()
         op = O.params(int(params.algorithm), params.weights, False)
         want, stats = O.compress(int(fmt), img, W, H, op, threads=threads, stats=True)
         bad = int(np.count_nonzero(np.any(gpu_out.reshape(-1, 16) != want.reshape(-1, 16), axis=1)))
@@ -592,10 +702,19 @@ def main():
             "value": round(W * 4 * sample_rows / 1e6 / t_cpu, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
             "sample": "full %dx%d BC3 IterativeClusterFit texture, median of %d runs, %d threads on %s"
                       % (W, 4 * sample_rows, len(times), threads, cpu_model()),
+            "cpu_model": cpu_model(), "cpus_online": online, "cpus_in_affinity": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "single_core_mpix_s": round(W * band / 1e6 / t_one, 3),
             "single_core_sample": "rows 0-%d of the same texture, 1 thread" % (band - 1),
         }
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+
+        # CPU halves of the secondaries
+        sec = line["secondary"]
+        for key, kind, g in pending:
+            row = bc1_cluster_cpu(O, g, threads) if kind == "bc1" else chain_cpu(O, chain, g, threads)
+            sec[key] = {k: v for k, v in row.items() if k != "config"}
+            print("bench: %s %s" % (key, json.dumps(sec[key])), file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 

@@ -10,8 +10,13 @@
  * Ownership: the caller owns every buffer; nothing is retained after a call
  * returns, except the library's per-device context (streams, scratch pools)
  * and the opaque batch/server handles, which the caller destroys.
- * Thread safety: every entry point may be called from any thread; calls on
- * one device are serialised on that device's context lock.
+ * Thread safety: every entry point may be called from any thread.  Calls
+ * with host buffers run synchronously and are serialised per device on that
+ * device's context lock.  Device-pointer calls only enqueue; where two calls
+ * share library-owned device memory (the binned pipeline's intermediate
+ * buffer, one batch handle's counters and records) the library orders their
+ * kernels with events, so they run one after another whatever streams they
+ * were issued on.
  * Errors: negative return codes (see TX_ERR_*).  Where the reference panics
  * (short input, short output, zero width) these return an error instead and
  * leave the output unspecified.
@@ -133,6 +138,11 @@ typedef struct {
     uint32_t pipeline;         /* 1: ClusterFit blocks through the binned pipeline (classify, sort by
                                   colour count, independent per-block searches); 0: one persistent
                                   kernel with the four blocks of a wave in lockstep */
+    uint32_t lanes_per_block;  /* lanes that encode one BC4/BC5 block: 1 (0 = default: a lane runs the
+                                  block's codebook fits alone, chunks of 512 blocks) or 16 (a 16-lane
+                                  group per block, min/max and error sums across the group, chunks of
+                                  chunk_blocks).  BC1-BC3 blocks use 16 lanes in the lockstep kernel
+                                  and the searches, one in the pipeline's classify pass */
 } tx_batch_tuning;
 
 typedef struct tx_batch tx_batch;
@@ -141,7 +151,9 @@ typedef struct tx_batch tx_batch;
 tx_batch_tuning tx_batch_tuning_default(void);
 /* Validates the textures, builds and uploads the chunk queue. */
 int tx_batch_create(tx_batch** batch, const tx_texture* textures, size_t count, const tx_batch_tuning* tuning);
-/* Enqueues one pass over every texture on `stream` (NULL = library stream). */
+/* Enqueues one pass over every texture on `stream` (NULL = library stream).
+ * Passes of one handle run one after another even when issued on different
+ * streams or threads (each waits for the previous one). */
 int tx_batch_run(tx_batch* batch, void* stream);
 /* Enqueues the pass and waits for it; returns the elapsed ms (HIP events). */
 int tx_batch_run_timed(tx_batch* batch, int iters, float* ms_total);
@@ -254,13 +266,6 @@ int tx_util_time_phases(int format, const uint8_t* d_rgba, size_t width, size_t 
  * paths give identical bytes.  0 = always the pipeline, UINT64_MAX = never.
  * Process-wide; returns the previous value (default 45056). */
 uint64_t tx_set_pipeline_threshold(uint64_t min_blocks);
-
-/* Measures the device's plain FP32 VALU rate (separate multiply and add with
- * vector-register operands, no FMA, no packed math) in lane-operations per
- * second, with 8 waves per SIMD; the _waves form runs 1..8 waves per SIMD
- * (the issue rate grows with the waves a SIMD can choose from). */
-int tx_util_valu_probe(double* lane_ops_per_s);
-int tx_util_valu_probe_waves(int waves_per_simd, double* lane_ops_per_s);
 
 /* Human-readable text for an error code. */
 const char* tx_error_string(int code);

@@ -65,3 +65,45 @@ def test_alpha_ties_texture(gpu, oracle, fmt):
     want = oracle.compress(fmt, img, w, h, oracle.params(int(p.algorithm), p.weights, False),
                            threads=oracle.default_threads())
     assert np.array_equal(got, want)
+
+
+def two_value_texture(seed):
+    """Every pair (a, b), a <= b, as one block holding a and b in a seeded
+    layout (R: a/b, G: b/a, A: a/b; B random), tiled row-major into a
+    texture of 184 x 179 blocks (the last blocks repeat pairs)."""
+    rng = np.random.default_rng(seed)
+    pairs = np.array([(a, b) for a in range(256) for b in range(a, 256)], np.int64)
+    bw, bh = 184, 179
+    n = bw * bh
+    pairs = np.concatenate([pairs, pairs[rng.integers(0, len(pairs), n - len(pairs))]])
+    pat = rng.integers(0, 2, size=(n, 16)).astype(bool)
+    pat[::7] = np.arange(16) < rng.integers(1, 16, size=(len(pat[::7]), 1))  # k copies of a, 16 - k of b
+    px = np.zeros((n, 16, 4), np.int64)
+    px[:, :, 0] = np.where(pat, pairs[:, :1], pairs[:, 1:])
+    px[:, :, 1] = np.where(pat, pairs[:, 1:], pairs[:, :1])
+    px[:, :, 2] = rng.integers(0, 256, size=(n, 16))
+    px[:, :, 3] = px[:, :, 0]
+    img = px.astype(np.uint8).reshape(bh, bw, 4, 4, 4).transpose(0, 2, 1, 3, 4).reshape(4 * bh, 4 * bw, 4)
+    return np.ascontiguousarray(img)
+
+
+@pytest.mark.parametrize("fmt", [2, 3, 4], ids=["bc3", "bc4", "bc5"])
+def test_alpha_two_value_blocks_texture(gpu, oracle, fmt):
+    # every two-value block through the texture kernels (BC4/BC5: one block
+    # per lane; BC3: the pipeline's classify pass, forced on), all pixels valid
+    img = two_value_texture(960 + fmt)
+    h, w = img.shape[:2]
+    # ClusterFit: BC3 colour goes through the pipeline (RangeFit would not)
+    p = tx.Params(tx.Algorithm.ClusterFit)
+    f = tx.Format(fmt)
+    old = tx.set_pipeline_threshold(0)
+    try:
+        got = np.zeros(f.compressed_size(w, h), np.uint8)
+        f.compress(img.reshape(-1), w, h, p, got)
+    finally:
+        tx.set_pipeline_threshold(old)
+    want = oracle.compress(fmt, img, w, h, oracle.params(int(p.algorithm), p.weights, False),
+                           threads=oracle.default_threads())
+    bs = f.block_size()
+    bad = np.nonzero(np.any(got.reshape(-1, bs) != want.reshape(-1, bs), axis=1))[0]
+    assert bad.size == 0, "%d blocks differ, first %d" % (bad.size, bad[0])

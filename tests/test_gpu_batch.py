@@ -83,8 +83,10 @@ def test_batch_parity_and_graph_replays(gpu, oracle):
 @pytest.mark.parametrize("tuning", [
     tx.Tuning(4, 1, 2, False, False), tx.Tuning(32, 2, 6, True, False), tx.Tuning(256, 4, 3, True, False),
     tx.Tuning(64, 1, 12, True, False), tx.Tuning(4, 1, 12, False, True), tx.Tuning(64, 1, 12, True, True),
-    tx.Tuning(256, 1, 12, True, True),
-], ids=["c4w1", "c32w2", "c256w4", "c64w1g12", "pipe-c4-nograph", "pipe-c64", "pipe-c256"])
+    tx.Tuning(256, 1, 12, True, True), tx.Tuning(32, 2, 6, True, False, 16), tx.Tuning(16, 4, 3, False, False, 16),
+    tx.Tuning(64, 1, 12, True, True, 16),
+], ids=["c4w1", "c32w2", "c256w4", "c64w1g12", "pipe-c4-nograph", "pipe-c64", "pipe-c256", "c32w2-alpha16",
+        "c16w4-alpha16-nograph", "pipe-c64-alpha16"])
 def test_batch_identical_across_tunings(gpu, oracle, tuning):
     items = make_set(9, 16, 70)
     bufs, descs = upload(items)

@@ -118,14 +118,17 @@ def test_wait_timeout_and_pending_bound(gpu):
     srv.close()
 
 
-@pytest.mark.parametrize("path", ["pipeline", "lockstep"])
+@pytest.mark.parametrize("path", ["pipeline", "lockstep", "pipeline-alpha16", "lockstep-alpha16"])
 def test_launch_paths_agree_with_the_oracle(gpu, oracle, path):
     # the same mixed jobs with every launch forced through the binned pipeline,
     # then through the lockstep kernel (alpha-only jobs use their own kernel in both)
-    old = tx.set_pipeline_threshold(0 if path == "pipeline" else 2 ** 64 - 1)
+    old = tx.set_pipeline_threshold(0 if path.startswith("pipeline") else 2 ** 64 - 1)
     try:
         jobs = make_jobs(7, 24, lo=4, hi=300)
-        srv = tx.Server(hbm_budget=512 << 20, max_inflight=2, max_jobs_per_launch=8)
+        kw = dict(hbm_budget=512 << 20, max_inflight=2, max_jobs_per_launch=8)
+        if path.endswith("alpha16"):
+            kw["tuning"] = tx.Tuning(64, 1, 12, False, True, 16)
+        srv = tx.Server(**kw)
         alone = run_alone(srv, jobs[:6])
         mixed, infos = run_mixed(srv, jobs)
         srv.close()

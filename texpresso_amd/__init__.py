@@ -68,6 +68,7 @@ class _TxTuning(ctypes.Structure):
         ("groups_per_cu", ctypes.c_uint32),
         ("use_graph", ctypes.c_uint32),
         ("pipeline", ctypes.c_uint32),
+        ("lanes_per_block", ctypes.c_uint32),
     ]
 
 
@@ -169,8 +170,6 @@ def lib():
             "tx_util_upload": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_download": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_copy": (c_int, [c_vp, c_vp, c_sz]),
-            "tx_util_valu_probe": (c_int, [ctypes.POINTER(ctypes.c_double)]),
-            "tx_util_valu_probe_waves": (c_int, [c_int, ctypes.POINTER(ctypes.c_double)]),
             "tx_set_pipeline_threshold": (ctypes.c_uint64, [ctypes.c_uint64]),
             "tx_util_time_compress": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
                                               ctypes.POINTER(ctypes.c_float)]),
@@ -442,25 +441,29 @@ def time_phases(fmt, d_rgba, width, height, params, d_out, out_len, iters):
 
 class Tuning:
     """Batch kernel tuning: blocks per queue chunk, waves (x4 blocks) per
-    workgroup, persistent workgroups per CU, hipGraph replay, and whether
-    ClusterFit blocks go through the binned pipeline."""
+    workgroup, persistent workgroups per CU, hipGraph replay, whether
+    ClusterFit blocks go through the binned pipeline, and the lanes that
+    encode one BC4/BC5 block (1 or 16)."""
 
-    __slots__ = ("chunk_blocks", "waves_per_group", "groups_per_cu", "use_graph", "pipeline")
+    __slots__ = ("chunk_blocks", "waves_per_group", "groups_per_cu", "use_graph", "pipeline", "lanes_per_block")
 
-    def __init__(self, chunk_blocks=64, waves_per_group=1, groups_per_cu=12, use_graph=True, pipeline=False):
+    def __init__(self, chunk_blocks=64, waves_per_group=1, groups_per_cu=12, use_graph=True, pipeline=False,
+                 lanes_per_block=1):
         self.chunk_blocks = int(chunk_blocks)
         self.waves_per_group = int(waves_per_group)
         self.groups_per_cu = int(groups_per_cu)
         self.use_graph = bool(use_graph)
         self.pipeline = bool(pipeline)
+        self.lanes_per_block = int(lanes_per_block)
 
     def _c(self):
         return _TxTuning(self.chunk_blocks, self.waves_per_group, self.groups_per_cu, 1 if self.use_graph else 0,
-                         1 if self.pipeline else 0)
+                         1 if self.pipeline else 0, self.lanes_per_block)
 
     @classmethod
     def _from_c(cls, t):
-        return cls(t.chunk_blocks, t.waves_per_group, t.groups_per_cu, bool(t.use_graph), bool(t.pipeline))
+        return cls(t.chunk_blocks, t.waves_per_group, t.groups_per_cu, bool(t.use_graph), bool(t.pipeline),
+                   t.lanes_per_block or 1)
 
     def as_dict(self):
         return {k: getattr(self, k) for k in self.__slots__}

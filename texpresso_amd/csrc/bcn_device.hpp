@@ -886,10 +886,89 @@ __device__ __forceinline__ void bc3_alpha_core(const Src& v, uint32_t valid, uin
     hi = (uint32_t)(idx >> 16);
 }
 
+// alpha.rs:187-256 for a block whose 16 pixels are all valid (every block
+// of an image but those of its last partial row and column), branch-free:
+//  - min5 is the minimum over the non-zero values and max5 the maximum over
+//    the values below 255 (255 and 0 when there are none, alpha.rs:192-218):
+//    (v + 255) & 255 maps 0 to 255 and v > 0 to v - 1, so min5 is one more
+//    than the minimum of those keys (capped at 255); (v + 1) & 255 maps 255
+//    to 0 and v < 255 to v + 1, so max5 is one less than their maximum
+//    (floored at 0).
+//  - after fix_range, max5 >= min5 + 5 and max7 >= min7 + 7, so
+//    write_alpha_block5 never swaps and write_alpha_block7 always does
+//    (alpha.rs:146-185): the 5-point block is (min5, max5, indices) and the
+//    7-point block (max7, min7, indices with 0 <-> 1 and x -> 9 - x).
+//  - indices are kept 8 to a word (3 bits each, the layout of
+//    write_alpha_block's two 24-bit groups, alpha.rs:120-144), and the
+//    7-point remap runs on all 8 fields of a word at once.
+// Same bytes as bc3_alpha_core with valid = 0xFFFF.
+__device__ __forceinline__ uint32_t remap7_fields(uint32_t x)
+{
+    // per 3-bit field: x < 2 -> x ^ 1, else 9 - x = (x ^ 7) + 2 (no carry out
+    // of the field, since x ^ 7 <= 5 there)
+    constexpr uint32_t kOnes = 0x249249u;  // bit 0 of each of the 8 fields
+    const uint32_t ge2 = ((x >> 1) | (x >> 2)) & kOnes;
+    const uint32_t hi_mask = ge2 * 7u;
+    return (((x ^ 0xFFFFFFu) + 2u * ge2) & hi_mask) | ((x ^ kOnes) & ~hi_mask & 0xFFFFFFu);
+}
+
+template <int U, class Src>
+__device__ __forceinline__ void bc3_alpha_full(const Src& v, uint32_t& lo, uint32_t& hi)
+{
+    uint32_t mn7 = 255u, mx7 = 0u, mn5k = 255u, mx5k = 0u;
+#pragma unroll U
+    for (int p = 0; p < 16; ++p) {
+        const uint32_t x = (uint32_t)v(p);
+        mn7 = min(mn7, x);
+        mx7 = max(mx7, x);
+        mn5k = min(mn5k, (x + 255u) & 255u);
+        mx5k = max(mx5k, (x + 1u) & 255u);
+    }
+    int min5 = (int)min(mn5k + 1u, 255u);
+    int max5 = max((int)mx5k - 1, 0);
+    int min7 = (int)mn7, max7 = (int)mx7;
+    if (min5 > max5) min5 = max5;
+    fix_range(min5, max5, 5);
+    fix_range(min7, max7, 7);
+    int codes5[8], codes7[8];
+    codes5[0] = min5;
+    codes5[1] = max5;
+#pragma unroll
+    for (int i = 1; i < 5; ++i) codes5[1 + i] = ((5 - i) * min5 + i * max5) / 5;
+    codes5[6] = 0;
+    codes5[7] = 255;
+    codes7[0] = min5;
+    codes7[1] = max5;
+#pragma unroll
+    for (int i = 1; i < 7; ++i) codes7[1 + i] = ((7 - i) * min7 + i * max7) / 7;
+    uint32_t i5[2] = {0u, 0u}, i7[2] = {0u, 0u};
+    uint32_t err5 = 0, err7 = 0;
+#pragma unroll U
+    for (int p = 0; p < 16; ++p) {
+        const int vp = v(p);
+        uint32_t l5, l7;
+        const uint32_t k5 = fit8(vp, codes5, l5), k7 = fit8(vp, codes7, l7);
+        err5 += l5;
+        err7 += l7;
+        i5[p >> 3] |= k5 << (3 * (p & 7));
+        i7[p >> 3] |= k7 << (3 * (p & 7));
+    }
+    uint32_t a0, a1, g0, g1;
+    if (err5 <= err7) {
+        a0 = (uint32_t)min5; a1 = (uint32_t)max5; g0 = i5[0]; g1 = i5[1];
+    } else {
+        a0 = (uint32_t)max7; a1 = (uint32_t)min7; g0 = remap7_fields(i7[0]); g1 = remap7_fields(i7[1]);
+    }
+    // bytes: a0, a1, then the two 24-bit index groups little-endian
+    lo = a0 | (a1 << 8) | (g0 << 16);
+    hi = (g0 >> 16) | (g1 << 8);
+}
+
 // The BC4 / BC5 channel form: the 16 values in registers.
 __device__ __forceinline__ void bc3_alpha_lane(const int (&v)[16], uint32_t valid, uint32_t& lo, uint32_t& hi)
 {
-    bc3_alpha_core<16>([&](int p) { return v[p]; }, valid, lo, hi);
+    if (valid == 0xFFFFu) bc3_alpha_full<16>([&](int p) { return v[p]; }, lo, hi);
+    else bc3_alpha_core<16>([&](int p) { return v[p]; }, valid, lo, hi);
 }
 
 // One block per lane: the 16 pixels of block (bx, by) as 16 dword loads

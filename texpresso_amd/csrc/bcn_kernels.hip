@@ -3,6 +3,7 @@
 
 #include "bcn_device.hpp"
 #include "bcn_kernels.hpp"
+#include "bcn_lane.hpp"
 
 #pragma clang fp contract(off)
 
@@ -19,7 +20,8 @@ __device__ __forceinline__ void store_block(uint8_t* out, uint64_t b, const uint
         *reinterpret_cast<uint4*>(out + b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-// Plain per-texture launch: one wave per four consecutive output blocks.
+// Plain per-texture launch: one wave per four consecutive output blocks;
+// a grid smaller than the texture's waves walks them grid-stride.
 template <int FMT, int ALG>
 __global__ __launch_bounds__(64) void k_encode(const uint8_t* __restrict__ rgba, uint32_t width, uint32_t height,
                                                uint32_t blocks_wide, uint64_t nblocks, uint8_t* __restrict__ out,
@@ -27,15 +29,18 @@ __global__ __launch_bounds__(64) void k_encode(const uint8_t* __restrict__ rgba,
 {
     __shared__ GroupLds lds[4];
     const int tid = (int)threadIdx.x, g = tid >> 4, l = tid & 15;
-    const uint64_t b = (uint64_t)blockIdx.x * 4u + (uint64_t)g;
-    const bool live = b < nblocks;
-    const uint32_t bx = live ? (uint32_t)(b % blocks_wide) : 0u;
-    const uint32_t by = live ? (uint32_t)(b / blocks_wide) : 0u;
-    bool valid;
-    const uint32_t px = fetch_pixel(rgba, width, height, bx, by, live, l, valid);
-    uint32_t w[4];
-    encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
-    if (live && l == 0) store_block<FMT>(out, b, w);
+    const uint64_t nwaves = (nblocks + 3) / 4;
+    for (uint64_t wv = blockIdx.x; wv < nwaves; wv += gridDim.x) {
+        const uint64_t b = wv * 4u + (uint64_t)g;
+        const bool live = b < nblocks;
+        const uint32_t bx = live ? (uint32_t)(b % blocks_wide) : 0u;
+        const uint32_t by = live ? (uint32_t)(b / blocks_wide) : 0u;
+        bool valid;
+        const uint32_t px = fetch_pixel(rgba, width, height, bx, by, live, l, valid);
+        uint32_t w[4];
+        encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
+        if (live && l == 0) store_block<FMT>(out, b, w);
+    }
 }
 
 // BC4 / BC5 per-texture launch: one block per lane (alpha codebooks are
@@ -46,14 +51,46 @@ __global__ __launch_bounds__(256) void k_encode_alpha(const uint8_t* __restrict_
                                                       uint32_t height, uint32_t blocks_wide, uint64_t nblocks,
                                                       uint8_t* __restrict__ out)
 {
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
-    uint32_t px[16];
-    const uint32_t valid = fetch_block_lane(rgba, width, height, (uint32_t)(b % blocks_wide),
-                                            (uint32_t)(b / blocks_wide), px);
-    uint32_t w[4];
-    encode_alpha_lane<FMT>(px, valid, w);
-    store_block<FMT>(out, b, w);
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblocks;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t px[16];
+        const uint32_t valid = fetch_block_lane(rgba, width, height, (uint32_t)(b % blocks_wide),
+                                                (uint32_t)(b / blocks_wide), px);
+        uint32_t w[4];
+        encode_alpha_lane<FMT>(px, valid, w);
+        store_block<FMT>(out, b, w);
+    }
+}
+
+// RangeFit per-texture launch (BC1/BC2/BC3): one block per lane, the front of
+// the pipeline's classify pass (alpha, colour set, principal axis, then the
+// single-colour or range fit) without its record path.  A RangeFit block is a
+// few hundred serial operations; on a 16-lane group every fold would run on
+// all 16 lanes.  Single-wave workgroups (the per-wave LDS copies of the
+// block pixels and weights), grid-stride over the texture's 64-block spans.
+template <int FMT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_encode_range_lane(
+    const uint8_t* __restrict__ rgba, uint32_t width, uint32_t height, uint32_t blocks_wide, uint64_t nblocks,
+    uint8_t* __restrict__ out, FitParams fp)
+{
+    __shared__ LaneTables LT;
+    __shared__ LaneBlockLds LB;
+    const int lane = (int)threadIdx.x;
+    fill_lane_tables(LT, lane);
+    group_sync();
+    for (uint64_t base = (uint64_t)blockIdx.x * 64u; base < nblocks; base += (uint64_t)gridDim.x * 64u) {
+        const uint64_t b = base + (uint64_t)lane;
+        if (b < nblocks) {
+            uint32_t px[16];
+            const uint32_t valid = fetch_block_lane(rgba, width, height, (uint32_t)(b % blocks_wide),
+                                                    (uint32_t)(b / blocks_wide), px);
+            uint32_t w[4];
+            LaneSet S;
+            float vx, vy, vz;
+            encode_front_lane<FMT, RANGE_FIT>(px, valid, fp, LT, LB, lane, w, S, vx, vy, vz);
+            store_block<FMT>(out, b, w);
+        }
+    }
 }
 
 // Explicit block list (Format::compress_block_masked over many blocks): block
@@ -65,14 +102,17 @@ __global__ __launch_bounds__(64) void k_encode_blocks(const uint8_t* __restrict_
 {
     __shared__ GroupLds lds[4];
     const int tid = (int)threadIdx.x, g = tid >> 4, l = tid & 15;
-    const uint64_t b = (uint64_t)blockIdx.x * 4u + (uint64_t)g;
-    const bool live = b < nblocks;
-    const uint32_t mask = live ? masks[b] : 0u;
-    const bool valid = live && ((mask >> l) & 1u);
-    const uint32_t px = valid ? reinterpret_cast<const uint32_t*>(blocks + 64 * b)[l] : 0u;
-    uint32_t w[4];
-    encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
-    if (live && l == 0) store_block<FMT>(out, b, w);
+    const uint64_t nwaves = (nblocks + 3) / 4;
+    for (uint64_t wv = blockIdx.x; wv < nwaves; wv += gridDim.x) {
+        const uint64_t b = wv * 4u + (uint64_t)g;
+        const bool live = b < nblocks;
+        const uint32_t mask = live ? masks[b] : 0u;
+        const bool valid = live && ((mask >> l) & 1u);
+        const uint32_t px = valid ? reinterpret_cast<const uint32_t*>(blocks + 64 * b)[l] : 0u;
+        uint32_t w[4];
+        encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
+        if (live && l == 0) store_block<FMT>(out, b, w);
+    }
 }
 
 // ------------------------------------------------------------------ decode
@@ -122,12 +162,9 @@ __device__ __forceinline__ uint32_t colour_value(const uint8_t* blk, bool is_bc1
 }
 
 template <int FMT>
-__global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ data, uint32_t width, uint32_t height,
-                                                uint32_t blocks_wide, uint8_t* __restrict__ out)
+__device__ __forceinline__ void decode_pixel(const uint8_t* __restrict__ data, uint32_t width, uint32_t x, uint32_t y,
+                                             uint32_t blocks_wide, uint8_t* __restrict__ out)
 {
-    const uint32_t x = blockIdx.x * 64u + (threadIdx.x & 63u);
-    const uint32_t y = blockIdx.y * 4u + (threadIdx.x >> 6);
-    if (x >= width || y >= height) return;
     constexpr uint32_t bs = (FMT == BC1 || FMT == BC4) ? 8u : 16u;
     const uint8_t* blk = data + ((size_t)(y >> 2) * blocks_wide + (x >> 2)) * bs;
     const int p = (int)((y & 3u) * 4u + (x & 3u));
@@ -153,6 +190,22 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ data
     *reinterpret_cast<uint32_t*>(out + 4 * ((size_t)y * width + x)) = rgba;
 }
 
+// One thread per pixel of a 64 x 4 tile; grids smaller than the image walk
+// its tiles grid-stride in both directions.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ data, uint32_t width, uint32_t height,
+                                                uint32_t blocks_wide, uint8_t* __restrict__ out)
+{
+    for (uint32_t ty = blockIdx.y; ty < (height + 3u) / 4u; ty += gridDim.y) {
+        const uint32_t y = ty * 4u + (threadIdx.x >> 6);
+        if (y >= height) continue;
+        for (uint32_t tx = blockIdx.x; tx < (width + 63u) / 64u; tx += gridDim.x) {
+            const uint32_t x = tx * 64u + (threadIdx.x & 63u);
+            if (x < width) decode_pixel<FMT>(data, width, x, y, blocks_wide, out);
+        }
+    }
+}
+
 }  // namespace txd
 
 // ---------------------------------------------------------------- launchers
@@ -164,8 +217,20 @@ hipError_t launch_encode_t(const txk::EncodeArgs& a, hipStream_t s)
 {
     const uint64_t groups = (a.nblocks + 3) / 4;
     if (groups == 0) return hipSuccess;
-    hipLaunchKernelGGL((txd::k_encode<FMT, ALG>), dim3((uint32_t)groups), dim3(64), 0, s, a.rgba, a.width, a.height,
-                       a.blocks_wide, a.nblocks, a.out, txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
+    hipLaunchKernelGGL((txd::k_encode<FMT, ALG>), dim3(txk::capped_grid(groups, txk::kMaxEncodeGroups)), dim3(64), 0,
+                       s, a.rgba, a.width, a.height, a.blocks_wide, a.nblocks, a.out,
+                       txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
+    return hipGetLastError();
+}
+
+template <int FMT>
+hipError_t launch_encode_range(const txk::EncodeArgs& a, hipStream_t s)
+{
+    const uint64_t waves = (a.nblocks + 63) / 64;
+    if (waves == 0) return hipSuccess;
+    hipLaunchKernelGGL((txd::k_encode_range_lane<FMT>), dim3(txk::capped_grid(waves, txk::kMaxEncodeGroups)), dim3(64),
+                       0, s, a.rgba, a.width, a.height, a.blocks_wide, a.nblocks, a.out,
+                       txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
     return hipGetLastError();
 }
 
@@ -173,7 +238,7 @@ template <int FMT>
 hipError_t launch_encode_f(const txk::EncodeArgs& a, hipStream_t s)
 {
     switch (a.algorithm) {
-    case 0: return launch_encode_t<FMT, 0>(a, s);
+    case 0: return launch_encode_range<FMT>(a, s);
     case 1: return launch_encode_t<FMT, 1>(a, s);
     default: return launch_encode_t<FMT, 2>(a, s);
     }
@@ -184,8 +249,9 @@ hipError_t launch_blocks_t(const txk::BlockListArgs& a, hipStream_t s)
 {
     const uint64_t groups = (a.nblocks + 3) / 4;
     if (groups == 0) return hipSuccess;
-    hipLaunchKernelGGL((txd::k_encode_blocks<FMT, ALG>), dim3((uint32_t)groups), dim3(64), 0, s, a.blocks, a.masks,
-                       a.nblocks, a.out, txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
+    hipLaunchKernelGGL((txd::k_encode_blocks<FMT, ALG>), dim3(txk::capped_grid(groups, txk::kMaxEncodeGroups)),
+                       dim3(64), 0, s, a.blocks, a.masks, a.nblocks, a.out,
+                       txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
     return hipGetLastError();
 }
 
@@ -204,7 +270,9 @@ hipError_t launch_decode_t(const uint8_t* data, uint32_t w, uint32_t h, uint8_t*
 {
     if (w == 0 || h == 0) return hipSuccess;
     const uint32_t bw = (w + 3) / 4;
-    hipLaunchKernelGGL((txd::k_decode<FMT>), dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, s, data, w, h, bw, out);
+    const dim3 grid(txk::capped_grid(((uint64_t)w + 63) / 64, txk::kMaxDecodeTilesX),
+                    txk::capped_grid(((uint64_t)h + 3) / 4, txk::kMaxDecodeTilesY));
+    hipLaunchKernelGGL((txd::k_decode<FMT>), grid, dim3(256), 0, s, data, w, h, bw, out);
     return hipGetLastError();
 }
 
@@ -217,9 +285,8 @@ hipError_t launch_encode_alpha(const EncodeArgs& a, hipStream_t s)
 {
     const uint64_t groups = (a.nblocks + 255) / 256;
     if (groups == 0) return hipSuccess;
-    if (groups > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((txd::k_encode_alpha<FMT>), dim3((uint32_t)groups), dim3(256), 0, s, a.rgba, a.width, a.height,
-                       a.blocks_wide, a.nblocks, a.out);
+    hipLaunchKernelGGL((txd::k_encode_alpha<FMT>), dim3(capped_grid(groups, kMaxAlphaGroups)), dim3(256), 0, s, a.rgba,
+                       a.width, a.height, a.blocks_wide, a.nblocks, a.out);
     return hipGetLastError();
 }
 

@@ -42,24 +42,55 @@ struct BatchTex {
 // Counter reset + persistent kernels over chunks {texture, first block,
 // block count, 0}: the first `nheavy` chunks (any format) go to the LDS
 // encoder (`grid` workgroups of `waves_per_group` = 1, 2 or 4 waves), the
-// next `nlight` (BC4/BC5 only) to the alpha-only kernel (`grid_light`
-// 4-wave workgroups), on `s_light` when given (fork/join through the two
-// events) or after the heavy kernel on `s`.  d_counter holds two counters.
+// next `nlight` (BC4/BC5 only) to the alpha-only launch (launch_alpha_chunks,
+// `grid_light` workgroups, `alpha_lanes` lanes per block), on `s_light` when
+// given (fork/join through the two events) or after the heavy kernel on `s`.
+// d_counter holds two counters.
 hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nheavy, uint32_t nlight,
                             uint32_t* d_counter, uint32_t grid, uint32_t grid_light, uint32_t waves_per_group,
-                            hipStream_t s, hipStream_t s_light, hipEvent_t fork, hipEvent_t join);
+                            uint32_t alpha_lanes, hipStream_t s, hipStream_t s_light, hipEvent_t fork,
+                            hipEvent_t join);
 
-// Blocks per chunk of an alpha-only (BC4/BC5) texture in the persistent
-// kernels: the alpha-only kernel runs one block per lane, 8 steps of 64.
+// Blocks per chunk of an alpha-only (BC4/BC5) texture encoded one block per
+// lane: the kernel runs 8 steps of 64 blocks per chunk.
 constexpr uint32_t kLightChunkBlocks = 512;
 
-// Counter reset (d_counter[0..1]) + the alpha-only persistent kernel over
-// chunks [first, first + nlight) on `s`.
+// Persistent kernel over the BC4/BC5 chunks [first, first + nlight):
+// alpha_lanes = 1 runs one block per lane (k_batch_light, 4-wave
+// workgroups); 16 runs a 16-lane group per block (k_batch with
+// `waves_per_group` waves per workgroup).  `d_counter` is its work counter
+// (not reset here).
+hipError_t launch_alpha_chunks(const BatchTex* d_tex, const uint4* d_chunks, uint32_t first, uint32_t nlight,
+                               uint32_t* d_counter, uint32_t grid, uint32_t alpha_lanes, uint32_t waves_per_group,
+                               hipStream_t s);
+
+// Workgroups of the alpha-only launch for `nlight` chunks on `cus` CUs.
+inline uint32_t alpha_grid(uint32_t cus, uint32_t nlight, uint32_t alpha_lanes, uint32_t groups_per_cu,
+                           uint32_t waves_per_group)
+{
+    const uint64_t want = alpha_lanes == 16 ? ((uint64_t)nlight + waves_per_group - 1) / waves_per_group
+                                            : ((uint64_t)nlight + 3) / 4;
+    const uint64_t cap = alpha_lanes == 16 ? (uint64_t)cus * groups_per_cu : (uint64_t)cus * 8;
+    return (uint32_t)(want < 1 ? 1 : want < cap ? want : cap);
+}
+
+// Counter reset (d_counter[0..1]) + the alpha-only kernel over chunks
+// [first, first + nlight) on `s`.
 hipError_t launch_batch_light(const BatchTex* d_tex, const uint4* d_chunks, uint32_t first, uint32_t nlight,
-                              uint32_t* d_counter, uint32_t grid_light, hipStream_t s);
+                              uint32_t* d_counter, uint32_t grid_light, uint32_t alpha_lanes,
+                              uint32_t waves_per_group, hipStream_t s);
 
 // Clears d_counter[0..1] on `s` (a kernel: replay-safe inside graphs).
 hipError_t reset_counters(uint32_t* d_counter, hipStream_t s);
+
+// Grid caps of the one-pass kernels: a launch never exceeds 2^32 work-items
+// in a dimension; larger inputs are walked grid-stride by the kernels.
+constexpr uint64_t kMaxEncodeGroups = 1ull << 22;   // 64-thread groups, 4 blocks each (16M blocks per pass)
+constexpr uint64_t kMaxAlphaGroups = 1ull << 20;    // 256-thread groups, one block per lane
+constexpr uint64_t kMaxDecodeTilesX = 1ull << 16;   // 64 x 4 pixel tiles per grid row
+constexpr uint64_t kMaxDecodeTilesY = 1ull << 14;
+
+inline uint32_t capped_grid(uint64_t groups, uint64_t cap) { return (uint32_t)(groups < cap ? groups : cap); }
 
 hipError_t launch_encode(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s);

@@ -350,7 +350,11 @@ __device__ __forceinline__ int encode_front_lane(const uint32_t (&px)[16], uint3
     if (FMT == BC2) bc2_alpha_lane(px, valid, out[0], out[1]);
     // BC3 alpha from the pixel copies in LDS, loops rolled (the unrolled
     // 16 x 8 codebook search would need many more registers)
-    if (FMT == BC3) bc3_alpha_core<1>([&](int p) { return (int)(S.pix(p) >> 24); }, valid, out[0], out[1]);
+    if (FMT == BC3) {
+        auto alpha = [&](int p) { return (int)(S.pix(p) >> 24); };
+        if (valid == 0xFFFFu) bc3_alpha_full<1>(alpha, out[0], out[1]);
+        else bc3_alpha_core<1>(alpha, valid, out[0], out[1]);
+    }
     const int kind = S.n == 1 ? KIND_SINGLE : (ALG == RANGE_FIT || S.n == 0) ? KIND_RANGE : KIND_CLUSTER;
     uint32_t lo = 0, hi = 0;
     vx = vy = vz = 0.0f;

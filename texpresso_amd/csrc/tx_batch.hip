@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -149,19 +150,46 @@ hipError_t reset_counters(uint32_t* d_counter, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t launch_alpha_chunks(const BatchTex* d_tex, const uint4* d_chunks, uint32_t first, uint32_t nlight,
+                               uint32_t* d_counter, uint32_t grid, uint32_t alpha_lanes, uint32_t waves_per_group,
+                               hipStream_t s)
+{
+    if (nlight == 0) return hipSuccess;
+    if (alpha_lanes != 16) {
+        hipLaunchKernelGGL(txd::k_batch_light, dim3(grid), dim3(256), 0, s, d_tex, d_chunks, first, nlight,
+                           d_counter);
+        return hipGetLastError();
+    }
+    // 16 lanes per block: the LDS encoder's persistent kernel over the
+    // alpha-only chunks (it dispatches BC4/BC5 chunks to the group code)
+    switch (waves_per_group) {
+    case 4:
+        hipLaunchKernelGGL(txd::k_batch<4>, dim3(grid), dim3(256), 0, s, d_tex, d_chunks + first, nlight, d_counter);
+        break;
+    case 2:
+        hipLaunchKernelGGL(txd::k_batch<2>, dim3(grid), dim3(128), 0, s, d_tex, d_chunks + first, nlight, d_counter);
+        break;
+    default:
+        hipLaunchKernelGGL(txd::k_batch<1>, dim3(grid), dim3(64), 0, s, d_tex, d_chunks + first, nlight, d_counter);
+        break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_light(const BatchTex* d_tex, const uint4* d_chunks, uint32_t first, uint32_t nlight,
-                              uint32_t* d_counter, uint32_t grid_light, hipStream_t s)
+                              uint32_t* d_counter, uint32_t grid_light, uint32_t alpha_lanes,
+                              uint32_t waves_per_group, hipStream_t s)
 {
     if (nlight == 0) return hipSuccess;
     hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
-    hipLaunchKernelGGL(txd::k_batch_light, dim3(grid_light), dim3(256), 0, s, d_tex, d_chunks, first, nlight,
-                       d_counter);
-    return hipGetLastError();
+    return launch_alpha_chunks(d_tex, d_chunks, first, nlight, d_counter, grid_light, alpha_lanes, waves_per_group,
+                               s);
 }
 
 hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32_t nheavy, uint32_t nlight,
                             uint32_t* d_counter, uint32_t grid, uint32_t grid_light, uint32_t waves_per_group,
-                            hipStream_t s, hipStream_t s_light, hipEvent_t fork, hipEvent_t join)
+                            uint32_t alpha_lanes, hipStream_t s, hipStream_t s_light, hipEvent_t fork,
+                            hipEvent_t join)
 {
     hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
     hipStream_t sl = s;
@@ -170,9 +198,9 @@ hipError_t launch_batch_raw(const BatchTex* d_tex, const uint4* d_chunks, uint32
             return hipErrorUnknown;
         sl = s_light;
     }
-    if (nlight)
-        hipLaunchKernelGGL(txd::k_batch_light, dim3(grid_light), dim3(256), 0, sl, d_tex, d_chunks, nheavy, nlight,
-                           d_counter + 1);
+    if (nlight && launch_alpha_chunks(d_tex, d_chunks, nheavy, nlight, d_counter + 1, grid_light, alpha_lanes,
+                                      waves_per_group, sl) != hipSuccess)
+        return hipErrorUnknown;
     if (nheavy) {
         switch (waves_per_group) {
         case 4:
@@ -215,6 +243,12 @@ struct tx_batch {
     uint32_t pipe_grid = 0;
     uint64_t pipe_slots = 0;  // record slots covered by the heavy chunks
     uint32_t classify_grid = 0;
+    // Runs of one handle share its counters and record buffers: `mu` guards
+    // the lazy graph capture, and each run's stream waits for the previous
+    // run (`done`), so runs on different streams execute one after another.
+    std::mutex mu;
+    hipEvent_t done = nullptr;
+    bool ran = false;
 };
 
 namespace {
@@ -241,8 +275,8 @@ hipError_t launch_batch(tx_batch* b, hipStream_t s)
 {
     if (!b->tuning.pipeline || b->nheavy == 0)
         return txk::launch_batch_raw(b->d_tex, b->d_chunks, b->nheavy, b->nchunks - b->nheavy, b->d_counter,
-                                     b->grid, b->grid_light, b->tuning.waves_per_group, s, b->light_stream, b->fork,
-                                     b->join);
+                                     b->grid, b->grid_light, b->tuning.waves_per_group, b->tuning.lanes_per_block,
+                                     s, b->light_stream, b->fork, b->join);
     // alpha-only chunks on the side stream, the binned pipeline on `s`
     const uint32_t nlight = b->nchunks - b->nheavy;
     hipLaunchKernelGGL(txd::k_reset_counter, dim3(1), dim3(1), 0, s, b->d_counter);
@@ -252,9 +286,9 @@ hipError_t launch_batch(tx_batch* b, hipStream_t s)
             return hipErrorUnknown;
         sl = b->light_stream;
     }
-    if (nlight)
-        hipLaunchKernelGGL(txd::k_batch_light, dim3(b->grid_light), dim3(256), 0, sl, b->d_tex, b->d_chunks,
-                           b->nheavy, nlight, b->d_counter + 1);
+    if (nlight && txk::launch_alpha_chunks(b->d_tex, b->d_chunks, b->nheavy, nlight, b->d_counter + 1, b->grid_light,
+                                           b->tuning.lanes_per_block, b->tuning.waves_per_group, sl) != hipSuccess)
+        return hipErrorUnknown;
     txp::PipeLaunch p{};
     p.d_tex = b->d_tex;
     p.d_chunks = b->d_chunks;
@@ -280,6 +314,7 @@ void free_batch(tx_batch* b)
     if (b->light_stream) (void)hipStreamDestroy(b->light_stream);
     if (b->pipe_mem) (void)hipFree(b->pipe_mem);
     if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
     if (b->join) (void)hipEventDestroy(b->join);
     if (b->d_tex) (void)hipFree(b->d_tex);
     if (b->d_chunks) (void)hipFree(b->d_chunks);
@@ -299,6 +334,7 @@ tx_batch_tuning tx_batch_tuning_default(void)
     t.groups_per_cu = 12;
     t.use_graph = 1;
     t.pipeline = 0;
+    t.lanes_per_block = 1;
     return t;
 }
 
@@ -312,6 +348,8 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
     if (t.chunk_blocks == 0 || t.chunk_blocks % 4 != 0) return TX_ERR_INVALID;
     if (t.waves_per_group != 1 && t.waves_per_group != 2 && t.waves_per_group != 4) return TX_ERR_INVALID;
     if (t.groups_per_cu == 0 || t.groups_per_cu > 64 || t.pipeline > 1) return TX_ERR_INVALID;
+    if (t.lanes_per_block == 0) t.lanes_per_block = 1;
+    if (t.lanes_per_block != 1 && t.lanes_per_block != 16) return TX_ERR_INVALID;
     int rc = TX_OK;
     txc::DeviceContext* ctx = txc::current_context(&rc);
     if (!ctx) return rc;
@@ -341,8 +379,8 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
         d.wg = s.params.weights[1];
         d.wb = s.params.weights[2];
         d.weigh_alpha = s.params.weigh_colour_by_alpha ? 1u : 0u;
-        // alpha-only textures: one block per lane, so larger chunks
-        const uint32_t cb = cost_class(s) == 0 ? txk::kLightChunkBlocks : t.chunk_blocks;
+        // alpha-only textures one block per lane take larger chunks
+        const uint32_t cb = (cost_class(s) == 0 && t.lanes_per_block == 1) ? txk::kLightChunkBlocks : t.chunk_blocks;
         for (uint32_t f = 0; f < d.nblocks; f += cb) {
             chunks.push_back(make_uint4((uint32_t)i, f, std::min(cb, d.nblocks - f), 0u));
             cls.push_back(cost_class(s));
@@ -375,7 +413,8 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
     for (size_t i = 0; i < order.size(); ++i) b->nheavy += cls[order[i]] > 0 ? 1u : 0u;
     b->nblocks = nblocks;
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
+    if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+        hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
         free_batch(b);
         return TX_ERR_HIP;
     }
@@ -383,10 +422,11 @@ int tx_batch_create(tx_batch** out, const tx_texture* textures, size_t count, co
     b->grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * t.groups_per_cu,
                                            std::max<uint64_t>(1, (b->nheavy + t.waves_per_group - 1) /
                                                                      t.waves_per_group));
-    // alpha-only kernel: 4-wave workgroups, up to 8 per CU
+    // alpha-only kernel: one block per lane in 4-wave workgroups (up to 8 per
+    // CU), or 16 lanes per block in the tuned workgroups
     const uint32_t nlight = b->nchunks - b->nheavy;
-    b->grid_light = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
-                                                 std::max<uint64_t>(1, (nlight + 3) / 4));
+    b->grid_light = txk::alpha_grid((uint32_t)prop.multiProcessorCount, nlight, t.lanes_per_block, t.groups_per_cu,
+                                    t.waves_per_group);
     if (nlight && b->nheavy) {
         if (hipStreamCreateWithFlags(&b->light_stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
@@ -435,15 +475,30 @@ int tx_batch_run(tx_batch* b, void* stream)
     if (ctx->device != b->device) return TX_ERR_INVALID;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     if (b->nchunks == 0) return TX_OK;
-    if (!b->tuning.use_graph) return launch_batch(b, s) == hipSuccess ? TX_OK : TX_ERR_HIP;
-    if (!b->exec) {
-        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return TX_ERR_HIP;
-        const hipError_t le = launch_batch(b, s);
-        const hipError_t ce = hipStreamEndCapture(s, &b->graph);
-        if (le != hipSuccess || ce != hipSuccess) return TX_ERR_HIP;
-        if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) return TX_ERR_HIP;
+    std::lock_guard<std::mutex> lock(b->mu);
+    if (b->ran && hipStreamWaitEvent(s, b->done, 0) != hipSuccess) return TX_ERR_HIP;
+    if (!b->tuning.use_graph) {
+        if (launch_batch(b, s) != hipSuccess) return TX_ERR_HIP;
+    } else {
+        if (!b->exec) {
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return TX_ERR_HIP;
+            const hipError_t le = launch_batch(b, s);
+            const hipError_t ce = hipStreamEndCapture(s, &b->graph);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (b->graph) (void)hipGraphDestroy(b->graph);
+                b->graph = nullptr;
+                return TX_ERR_HIP;
+            }
+            if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+                b->exec = nullptr;
+                return TX_ERR_HIP;
+            }
+        }
+        if (hipGraphLaunch(b->exec, s) != hipSuccess) return TX_ERR_HIP;
     }
-    return hipGraphLaunch(b->exec, s) == hipSuccess ? TX_OK : TX_ERR_HIP;
+    if (hipEventRecord(b->done, s) != hipSuccess) return TX_ERR_HIP;
+    b->ran = true;
+    return TX_OK;
 }
 
 int tx_batch_run_timed(tx_batch* b, int iters, float* ms_total)
@@ -474,6 +529,11 @@ int tx_batch_destroy(tx_batch* b)
     int rc = TX_OK;
     txc::DeviceContext* ctx = txc::current_context(&rc);
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    {
+        // the last run, on whichever stream it was enqueued
+        std::lock_guard<std::mutex> lock(b->mu);
+        if (b->ran) (void)hipEventSynchronize(b->done);
+    }
     free_batch(b);
     return TX_OK;
 }
@@ -491,6 +551,8 @@ int tx_batch_autotune(const tx_texture* textures, size_t count, const tx_batch_t
 {
     if (!best || !best_ms || iters < 1) return TX_ERR_INVALID;
     std::vector<tx_batch_tuning> grid;
+    bool alpha_only = false;  // BC4/BC5 textures present: lanes per block is tuned too
+    for (size_t i = 0; i < count && textures; ++i) alpha_only = alpha_only || textures[i].format >= TX_BC4;
     if (candidates && ncand) {
         grid.assign(candidates, candidates + ncand);
     } else {
@@ -505,7 +567,7 @@ int tx_batch_autotune(const tx_texture* textures, size_t count, const tx_batch_t
             grid.push_back(p);
             for (uint32_t w : wpg)
                 for (uint32_t v : waves_per_cu) {
-                    tx_batch_tuning t;
+                    tx_batch_tuning t = tx_batch_tuning_default();
                     t.chunk_blocks = c;
                     t.waves_per_group = w;
                     t.groups_per_cu = std::max(1u, v / w);
@@ -513,6 +575,14 @@ int tx_batch_autotune(const tx_texture* textures, size_t count, const tx_batch_t
                     t.pipeline = 0;
                     grid.push_back(t);
                 }
+        }
+        if (alpha_only) {
+            const size_t n = grid.size();
+            for (size_t i = 0; i < n; ++i) {
+                tx_batch_tuning t = grid[i];
+                t.lanes_per_block = 16;
+                grid.push_back(t);
+            }
         }
     }
     float best_t = -1.0f;

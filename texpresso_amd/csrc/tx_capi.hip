@@ -139,24 +139,18 @@ static uint64_t pipeline_threshold(int format, int algorithm)
     return t > UINT64_MAX / quarters ? UINT64_MAX : t * quarters / 4;
 }
 
-// Encodes one texture on `s`: ClusterFit textures of at least
-// pipeline_threshold() blocks go through the binned pipeline (classify,
-// bin, independent per-block searches), everything else through the direct
-// kernel.  Both produce the same bytes.  With `marks` (5 events) the
-// pipeline records them between its kernels; `used_pipeline` says which
-// path ran.
-static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, hipStream_t s,
-                            hipEvent_t* marks = nullptr, bool* used_pipeline = nullptr)
+// Whether a texture goes through the binned pipeline: ClusterFit or
+// IterativeClusterFit in BC1-BC3 with at least pipeline_threshold() blocks.
+static bool takes_pipeline(const txk::EncodeArgs& a)
 {
     const bool cluster = a.format <= TX_BC3 && a.algorithm != TX_RANGE_FIT;
-    if (used_pipeline) *used_pipeline = false;
-    if (!cluster || a.nblocks < pipeline_threshold(a.format, a.algorithm) || a.nblocks > 0xFFFFFFFFull)
-        return txk::launch_encode(a, s);
-    if (used_pipeline) *used_pipeline = true;
-    std::lock_guard<std::mutex> lock(ctx->pipe_mu);
-    txp::PipeBufs bufs;
-    if (!ctx->pipe_scratch(a.nblocks, &bufs)) return hipErrorOutOfMemory;
-    if (hipStreamWaitEvent(s, ctx->pipe_done, 0) != hipSuccess) return hipErrorUnknown;
+    return cluster && a.nblocks >= pipeline_threshold(a.format, a.algorithm) && a.nblocks <= 0xFFFFFFFFull;
+}
+
+// Launch shape of the binned pipeline for one texture on a device with
+// `cus` compute units.
+static txp::PipeLaunch pipe_launch_for(const txk::EncodeArgs& a, uint32_t cus)
+{
     txp::PipeLaunch p{};
     p.d_tex = nullptr;
     p.tex0.rgba = a.rgba;
@@ -176,9 +170,9 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
     // with fewer than 64 blocks per SIMD is cut into chunks of at least 16
     // so that it still reaches every SIMD (one wave per SIMD with all lanes
     // busy beats several part-filled ones)
-    const uint64_t cwaves = (uint64_t)ctx->cus * txp::kClassifyWavesPerCu;
-    const uint64_t swaves = (uint64_t)ctx->cus * txp::kSearchWavesPerCu;
-    const uint64_t simds = (uint64_t)ctx->cus * 4;
+    const uint64_t cwaves = (uint64_t)cus * txp::kClassifyWavesPerCu;
+    const uint64_t swaves = (uint64_t)cus * txp::kSearchWavesPerCu;
+    const uint64_t simds = (uint64_t)cus * 4;
     uint64_t cb = (a.nblocks + simds - 1) / simds;
     cb = std::min<uint64_t>(64, std::max<uint64_t>(16, (cb + 15) & ~(uint64_t)15));
     p.chunk_blocks = (uint32_t)cb;
@@ -188,7 +182,25 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
     p.classify_grid = (uint32_t)std::min<uint64_t>(cwaves, p.nchunks);
     p.search_grid = (uint32_t)std::min<uint64_t>(swaves, (a.nblocks + 3) / 4);
     p.lists = txp::lists_for((uint32_t)a.format, (uint32_t)a.algorithm);
-    const hipError_t e = txp::launch_pipeline(p, bufs, s, marks);
+    return p;
+}
+
+// Encodes one texture on `s`: textures that take_pipeline() go through the
+// binned pipeline (classify, bin, independent per-block searches),
+// everything else through the direct kernel.  Both produce the same bytes.
+// With `marks` (5 events) the pipeline records them between its kernels;
+// `used_pipeline` says which path ran.
+static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, hipStream_t s,
+                            hipEvent_t* marks = nullptr, bool* used_pipeline = nullptr)
+{
+    if (used_pipeline) *used_pipeline = false;
+    if (!takes_pipeline(a)) return txk::launch_encode(a, s);
+    if (used_pipeline) *used_pipeline = true;
+    std::lock_guard<std::mutex> lock(ctx->pipe_mu);
+    txp::PipeBufs bufs;
+    if (!ctx->pipe_scratch(a.nblocks, &bufs)) return hipErrorOutOfMemory;
+    if (hipStreamWaitEvent(s, ctx->pipe_done, 0) != hipSuccess) return hipErrorUnknown;
+    const hipError_t e = txp::launch_pipeline(pipe_launch_for(a, (uint32_t)ctx->cus), bufs, s, marks);
     if (hipEventRecord(ctx->pipe_done, s) != hipSuccess) return hipErrorUnknown;
     return e;
 }

@@ -97,7 +97,8 @@ struct Job {
     tx_job_info info{};
     size_t in_off = 0, in_bytes = 0, out_off = 0, out_bytes = 0;
     bool has_memory = false;
-    bool release_when_done = false;
+    bool release_when_done = false;  // release requested while running or being fetched
+    uint32_t fetching = 0;           // tx_server_fetch copies in progress from the job's output
 };
 
 struct Launch {
@@ -138,6 +139,7 @@ struct tx_server {
     bool stop = false;
     std::thread worker;
     hipStream_t fetch_stream = nullptr;
+    std::mutex fetch_mu;  // serialises copies on fetch_stream
 
     void run();
     bool start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_lock<std::mutex>& lock);
@@ -148,6 +150,16 @@ struct tx_server {
             arena.free(j.out_off, j.out_bytes);
             j.has_memory = false;
         }
+    }
+    // Frees and forgets a finished job whose release was requested, once no
+    // fetch reads its output any more; call with `mu` held.
+    void release_if_requested(std::unordered_map<uint64_t, Job>::iterator it)
+    {
+        Job& j = it->second;
+        if (!j.release_when_done || j.fetching) return;
+        free_job_memory(j);
+        jobs.erase(it);
+        cv_work.notify_one();
     }
 };
 
@@ -209,6 +221,10 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
     uint64_t want = (colour_blocks + waves - 1) / std::max<uint64_t>(1, waves);
     want = std::max<uint64_t>(4, (want + 3) & ~(uint64_t)3);
     const uint32_t chunk = (uint32_t)std::min<uint64_t>(cfg.tuning.chunk_blocks, want);
+    // BC4/BC5 jobs: one block per lane in chunks of 512, or 16 lanes per block
+    // in the colour chunks' size
+    const uint32_t alpha_lanes = cfg.tuning.lanes_per_block == 16 ? 16u : 1u;
+    const uint32_t light_chunk = alpha_lanes == 16 ? chunk : txk::kLightChunkBlocks;
     const bool pipeline = cfg.tuning.pipeline && colour_blocks >= txc::pipeline_min_blocks() &&
                           colour_blocks <= kServerPipeMaxBlocks;
     size_t nchunks = 0;
@@ -217,7 +233,7 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
     for (uint64_t id : ids) {
         Job& j = jobs[id];
         const uint64_t nb = (uint64_t)((j.spec.width + 3) / 4) * ((j.spec.height + 3) / 4);
-        const uint32_t cb = j.spec.format >= TX_BC4 ? txk::kLightChunkBlocks : chunk;
+        const uint32_t cb = j.spec.format >= TX_BC4 ? light_chunk : chunk;
         nchunks += (size_t)((nb + cb - 1) / cb);
         specs.push_back(j.spec);
         in_offs.push_back(j.in_off);
@@ -271,7 +287,7 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
         const size_t k = order[q];
         const txk::BatchTex& t = L.h_tex[k];
         const bool light = t.variant >= 9u;
-        const uint32_t cb = light ? txk::kLightChunkBlocks : chunk;
+        const uint32_t cb = light ? light_chunk : chunk;
         for (uint32_t f = 0; f < t.nblocks; f += cb) {
             const uint32_t n = std::min(cb, t.nblocks - f);
             L.h_chunks[c++] = make_uint4((uint32_t)k, f, n, light ? 0u : slot);
@@ -291,9 +307,11 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
     // no more persistent workgroups than chunks
     const uint32_t launch_grid = (uint32_t)std::min<uint64_t>(
         grid, std::max<uint64_t>(1, (nheavy + cfg.tuning.waves_per_group - 1) / cfg.tuning.waves_per_group));
-    const uint32_t light_grid = (uint32_t)std::min<uint64_t>((uint64_t)cus * 8, std::max<uint32_t>(1, (nlight + 3) / 4));
+    const uint32_t light_grid = txk::alpha_grid(cus, nlight, alpha_lanes, cfg.tuning.groups_per_cu,
+                                                cfg.tuning.waves_per_group);
     if (ok && use_pipe) {
-        if (txk::launch_batch_light(L.d_tex, L.d_chunks, nheavy, nlight, L.d_counter + 2, light_grid, L.stream) !=
+        if (txk::launch_batch_light(L.d_tex, L.d_chunks, nheavy, nlight, L.d_counter + 2, light_grid, alpha_lanes,
+                                    cfg.tuning.waves_per_group, L.stream) !=
             hipSuccess)
             ok = false;
         txp::PipeLaunch p{};
@@ -308,16 +326,20 @@ bool tx_server::start_launch(Launch& L, std::vector<uint64_t>& ids, std::unique_
         if (ok && txp::launch_pipeline(p, L.pipe, L.stream) != hipSuccess) ok = false;
     } else if (ok && c &&
                txk::launch_batch_raw(L.d_tex, L.d_chunks, nheavy, nlight, L.d_counter, launch_grid, light_grid,
-                                     cfg.tuning.waves_per_group, L.stream, nullptr, nullptr, nullptr) != hipSuccess) {
+                                     cfg.tuning.waves_per_group, alpha_lanes, L.stream, nullptr, nullptr,
+                                     nullptr) != hipSuccess) {
         ok = false;
     }
     if (ok && hipEventRecord(L.done, L.stream) != hipSuccess) ok = false;
     lock.lock();
     if (!ok) {
         for (uint64_t id : ids) {
-            Job& j = jobs[id];
-            j.info.status = TX_ERR_HIP;
-            j.info.t_done_ns = now_ns();
+            auto it = jobs.find(id);
+            if (it == jobs.end()) continue;
+            it->second.info.status = TX_ERR_HIP;
+            it->second.info.t_done_ns = now_ns();
+            // a release requested while the lock was dropped frees it now
+            release_if_requested(it);
         }
         (void)hipStreamSynchronize(L.stream);
         L.busy = false;
@@ -348,10 +370,7 @@ void tx_server::run()
                 if (it == jobs.end()) continue;
                 it->second.info.status = q == hipSuccess ? TX_JOB_DONE : TX_ERR_HIP;
                 it->second.info.t_done_ns = t;
-                if (it->second.release_when_done) {
-                    free_job_memory(it->second);
-                    jobs.erase(it);
-                }
+                release_if_requested(it);
             }
             L.jobs.clear();
             L.busy = false;
@@ -420,7 +439,8 @@ int tx_server_create(tx_server** out, const tx_server_config* config)
         return TX_ERR_INVALID;
     if (cfg.tuning.chunk_blocks == 0 || cfg.tuning.chunk_blocks % 4 ||
         (cfg.tuning.waves_per_group != 1 && cfg.tuning.waves_per_group != 2 && cfg.tuning.waves_per_group != 4) ||
-        cfg.tuning.groups_per_cu == 0 || cfg.tuning.groups_per_cu > 64 || cfg.tuning.pipeline > 1)
+        cfg.tuning.groups_per_cu == 0 || cfg.tuning.groups_per_cu > 64 || cfg.tuning.pipeline > 1 ||
+        (cfg.tuning.lanes_per_block != 0 && cfg.tuning.lanes_per_block != 1 && cfg.tuning.lanes_per_block != 16))
         return TX_ERR_INVALID;
     int rc = TX_OK;
     txc::DeviceContext* ctx = txc::current_context(&rc);
@@ -533,14 +553,31 @@ int tx_server_fetch(tx_server* s, uint64_t id, uint8_t* out, size_t out_len)
     {
         std::lock_guard<std::mutex> lock(s->mu);
         auto it = s->jobs.find(id);
-        if (it == s->jobs.end() || it->second.info.status != TX_JOB_DONE) return TX_ERR_INVALID;
+        if (it == s->jobs.end() || it->second.info.status != TX_JOB_DONE || it->second.release_when_done)
+            return TX_ERR_INVALID;
         src = it->second.info.d_output;
         n = it->second.info.out_len;
+        if (out_len < n) return TX_ERR_OUTPUT_SHORT;
+        // the output region stays allocated until this copy ends, even if the
+        // job is released meanwhile (the release is deferred to the end)
+        ++it->second.fetching;
     }
-    if (out_len < n) return TX_ERR_OUTPUT_SHORT;
     (void)hipSetDevice(s->device);
-    if (hipMemcpyAsync(out, src, n, hipMemcpyDeviceToHost, s->fetch_stream) != hipSuccess) return TX_ERR_HIP;
-    return hipStreamSynchronize(s->fetch_stream) == hipSuccess ? TX_OK : TX_ERR_HIP;
+    // one copy at a time on the fetch stream, so a failing copy's error is its own
+    int rc = TX_OK;
+    {
+        std::lock_guard<std::mutex> copy_lock(s->fetch_mu);
+        if (hipMemcpyAsync(out, src, n, hipMemcpyDeviceToHost, s->fetch_stream) != hipSuccess ||
+            hipStreamSynchronize(s->fetch_stream) != hipSuccess)
+            rc = TX_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(id);
+    if (it != s->jobs.end()) {
+        --it->second.fetching;
+        s->release_if_requested(it);
+    }
+    return rc;
 }
 
 int tx_server_release(tx_server* s, uint64_t id)
@@ -551,9 +588,9 @@ int tx_server_release(tx_server* s, uint64_t id)
     if (it == s->jobs.end()) return TX_ERR_INVALID;
     Job& j = it->second;
     if (j.info.status == TX_JOB_DONE || j.info.status < 0) {
-        s->free_job_memory(j);
-        s->jobs.erase(it);
-        s->cv_work.notify_one();
+        // freed now, or when the fetches still copying its output end
+        j.release_when_done = true;
+        s->release_if_requested(it);
         return TX_OK;
     }
     if (j.info.status == TX_JOB_PENDING) {
