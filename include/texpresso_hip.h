@@ -57,6 +57,7 @@ typedef struct {
 #define TX_ERR_BUSY (-9)           /* serving queue full / admission refused */
 #define TX_ERR_INVALID (-10)       /* bad handle or argument */
 #define TX_ERR_NOMEM (-11)         /* device memory budget exceeded */
+#define TX_ERR_TOO_LARGE (-12)     /* texture exceeds the fixed scratch of tx_compress_device_noalloc */
 
 /* Params::default(), lib/src/lib.rs:92-100: ClusterFit, perceptual weights
  * [0.2126, 0.7152, 0.0722] (lib/src/lib.rs:74), no alpha weighting. */
@@ -112,6 +113,30 @@ int tx_compress_device(int format, const uint8_t* d_rgba, size_t width, size_t h
                        uint8_t* d_output, size_t out_len, void* stream);
 int tx_decompress_device(int format, const uint8_t* d_data, size_t width, size_t height, uint8_t* d_output,
                          void* stream);
+
+/* Allocation-free variant of tx_compress_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and output bytes as tx_compress_device, except:
+ *  - no hipMalloc, hipFree or other allocating call is made, and no stream
+ *    or event is created; `stream` NULL means the null stream;
+ *  - the binned ClusterFit pipeline's intermediate buffer (records, keys,
+ *    sorted lists, counters) is a fixed device array inside the library,
+ *    TX_NOALLOC_SCRATCH_BYTES on every GPU the library runs on, held until
+ *    the process ends, instead of the grown per-device allocation; a
+ *    texture that takes the pipeline (tx_set_pipeline_threshold) may have at
+ *    most TX_NOALLOC_MAX_PIPELINE_BLOCKS blocks, larger ones return
+ *    TX_ERR_TOO_LARGE before anything is enqueued.  Textures below the
+ *    threshold, RangeFit and BC4/BC5 use no scratch and have no such limit.
+ * All calls on one GPU share that GPU's array and the library does not order
+ * them: calls that may overlap on one GPU (issued on different streams, or
+ * from threads with no ordering between them) race and corrupt each other's
+ * output.  Calls on one stream are ordered by the stream.  The array keeps
+ * the last call's contents; every call clears the counters it needs. */
+#define TX_NOALLOC_MAX_PIPELINE_BLOCKS 262144u /* a 2048 x 2048 texture */
+#define TX_NOALLOC_SCRATCH_BYTES 92275200u     /* 88 MiB: 352 bytes per block plus the counters */
+int tx_compress_device_noalloc(int format, const uint8_t* d_rgba, size_t width, size_t height,
+                               const tx_params* params, uint8_t* d_output, size_t out_len, void* stream);
 
 /* ---------------------------------------------------------------------
  * Batched encoding (no single reference counterpart: it runs many

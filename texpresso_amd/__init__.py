@@ -164,6 +164,7 @@ def lib():
             "tx_decompress_block": (c_int, [c_int, c_u8p, c_sz, c_u8p]),
             "tx_compress_device": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_vp]),
             "tx_decompress_device": (c_int, [c_int, c_vp, c_sz, c_sz, c_vp, c_vp]),
+            "tx_compress_device_noalloc": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_vp]),
             "tx_single_lut": (None, [c_u8p]),
             "tx_util_malloc": (c_int, [ctypes.POINTER(ctypes.c_void_p), c_sz]),
             "tx_util_free": (c_int, [c_vp]),

@@ -32,6 +32,52 @@ bool image_bytes(size_t w, size_t h, size_t* out)
 
 }  // namespace
 
+// Device scratch of tx_compress_device_noalloc: the binned pipeline's
+// records, keys, sorted lists and counters for TX_NOALLOC_MAX_PIPELINE_BLOCKS
+// blocks.  Every GPU the library's code object is loaded on holds its own
+// copy.  In host code `scratch` names a host-side stand-in, not a device
+// address: the per-device address comes from hipGetSymbolAddress
+// (NoallocDevice::ptr), and only that address is passed to kernels.
+static_assert(txp::PipeBufs::bytes_for(TX_NOALLOC_MAX_PIPELINE_BLOCKS) == TX_NOALLOC_SCRATCH_BYTES,
+              "TX_NOALLOC_SCRATCH_BYTES holds the pipeline buffers of TX_NOALLOC_MAX_PIPELINE_BLOCKS blocks");
+static __device__ __align__(256) uint8_t scratch[TX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+struct NoallocDevice {
+    bool ready = false;
+    uint8_t* ptr = nullptr;  // this device's copy of `scratch`
+    uint32_t cus = 0;
+};
+
+constexpr int kNoallocDevices = 64;
+std::mutex g_noalloc_mu;
+NoallocDevice g_noalloc[kNoallocDevices];
+
+// Set-up of the calling thread's current device, once per device: its
+// address of `scratch` and its compute-unit count.  Allocates nothing.
+int noalloc_device(const NoallocDevice** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoallocDevices) return TX_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoallocDevice& d = g_noalloc[dev];
+    if (!d.ready) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess || strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return TX_ERR_NO_DEVICE;
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return TX_ERR_HIP;
+        d.ptr = static_cast<uint8_t*>(p);
+        d.cus = (uint32_t)prop.multiProcessorCount;
+        d.ready = true;
+    }
+    *out = &d;
+    return TX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 tx_params tx_params_default(void)
@@ -73,6 +119,7 @@ const char* tx_error_string(int code)
     case TX_ERR_BUSY: return "queue full";
     case TX_ERR_INVALID: return "invalid argument or handle";
     case TX_ERR_NOMEM: return "device memory budget exceeded";
+    case TX_ERR_TOO_LARGE: return "texture exceeds the allocation-free scratch (TX_NOALLOC_MAX_PIPELINE_BLOCKS)";
     default: return "unknown error";
     }
 }
@@ -247,6 +294,29 @@ int tx_compress_device(int format, const uint8_t* d_rgba, size_t width, size_t h
     if (encode_on(ctx, encode_args(format, params, d_rgba, width, height, d_output, out_len), s) != hipSuccess)
         return TX_ERR_HIP;
     return TX_OK;
+}
+
+// ------------------------------------------------------------ allocation-free entry point
+
+int tx_compress_device_noalloc(int format, const uint8_t* d_rgba, size_t width, size_t height,
+                               const tx_params* params, uint8_t* d_output, size_t out_len, void* stream)
+{
+    size_t in_bytes = 0;
+    int rc = check_compress(format, width, height, params, out_len, &in_bytes);
+    if (rc != TX_OK) return rc;
+    if (out_len == 0) return TX_OK;
+    if ((in_bytes && !d_rgba) || !d_output) return TX_ERR_INVALID;
+    if (((uintptr_t)d_rgba & 3u) || ((uintptr_t)d_output & (tx_block_size(format) - 1))) return TX_ERR_ALIGN;
+    const NoallocDevice* dev = nullptr;
+    rc = noalloc_device(&dev);
+    if (rc != TX_OK) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    const txk::EncodeArgs a = encode_args(format, params, d_rgba, width, height, d_output, out_len);
+    if (!takes_pipeline(a)) return txk::launch_encode(a, s) == hipSuccess ? TX_OK : TX_ERR_HIP;
+    if (a.nblocks > TX_NOALLOC_MAX_PIPELINE_BLOCKS) return TX_ERR_TOO_LARGE;
+    txp::PipeBufs bufs;
+    bufs.carve(dev->ptr, TX_NOALLOC_MAX_PIPELINE_BLOCKS);
+    return txp::launch_pipeline(pipe_launch_for(a, dev->cus), bufs, s) == hipSuccess ? TX_OK : TX_ERR_HIP;
 }
 
 // ------------------------------------------------------------ mip chains

@@ -11,12 +11,6 @@
 
 namespace txp {
 
-size_t PipeBufs::bytes_for(uint64_t cap)
-{
-    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return up(sizeof(ClusterRec) * cap) + (1 + kLists) * up(4 * cap) + up(sizeof(PipeCtl));
-}
-
 void PipeBufs::carve(uint8_t* base, uint64_t cap)
 {
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };

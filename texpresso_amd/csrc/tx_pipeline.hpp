@@ -68,7 +68,11 @@ struct PipeBufs {
     uint64_t capacity = 0;
 
     // bytes of device memory for `cap` records (one allocation, 256-aligned parts)
-    static size_t bytes_for(uint64_t cap);
+    static constexpr size_t bytes_for(uint64_t cap)
+    {
+        return align256(sizeof(ClusterRec) * cap) + (1 + kLists) * align256(4 * cap) + align256(sizeof(PipeCtl));
+    }
+    static constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
     // carves the parts out of one allocation of bytes_for(cap) bytes
     void carve(uint8_t* base, uint64_t cap);
 };
