@@ -941,17 +941,32 @@ __device__ __forceinline__ void bc3_alpha_full(const Src& v, uint32_t& lo, uint3
     codes7[1] = max5;
 #pragma unroll
     for (int i = 1; i < 7; ++i) codes7[1 + i] = ((7 - i) * min7 + i * max7) / 7;
+    // per pixel and book: the key 8|v - code| + j of the winning code (fit8's
+    // keys; slots 0-1 are min5/max5 in both books, so their keys are shared),
+    // its distance squared into the error, its slot into the index word
+    uint32_t c5[8], c7[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        c5[j] = (uint32_t)codes5[j] << 3;
+        c7[j] = (uint32_t)codes7[j] << 3;
+    }
     uint32_t i5[2] = {0u, 0u}, i7[2] = {0u, 0u};
     uint32_t err5 = 0, err7 = 0;
 #pragma unroll U
     for (int p = 0; p < 16; ++p) {
-        const int vp = v(p);
-        uint32_t l5, l7;
-        const uint32_t k5 = fit8(vp, codes5, l5), k7 = fit8(vp, codes7, l7);
-        err5 += l5;
-        err7 += l7;
-        i5[p >> 3] |= k5 << (3 * (p & 7));
-        i7[p >> 3] |= k7 << (3 * (p & 7));
+        const uint32_t v8 = (uint32_t)v(p) << 3;
+        const uint32_t k01 = min(__builtin_amdgcn_sad_u16(v8, c5[0], 0u), __builtin_amdgcn_sad_u16(v8, c5[1], 1u));
+        uint32_t k5 = k01, k7 = k01;
+#pragma unroll
+        for (int j = 2; j < 8; ++j) {
+            k5 = min(k5, __builtin_amdgcn_sad_u16(v8, c5[j], (uint32_t)j));
+            k7 = min(k7, __builtin_amdgcn_sad_u16(v8, c7[j], (uint32_t)j));
+        }
+        const uint32_t d5 = __builtin_amdgcn_ubfe(k5, 3, 8), d7 = __builtin_amdgcn_ubfe(k7, 3, 8);
+        err5 = __umul24(d5, d5) + err5;
+        err7 = __umul24(d7, d7) + err7;
+        i5[p >> 3] |= (k5 & 7u) << (3 * (p & 7));
+        i7[p >> 3] |= (k7 & 7u) << (3 * (p & 7));
     }
     uint32_t a0, a1, g0, g1;
     if (err5 <= err7) {

@@ -20,8 +20,11 @@ __device__ __forceinline__ void store_block(uint8_t* out, uint64_t b, const uint
         *reinterpret_cast<uint4*>(out + b * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-// Plain per-texture launch: one wave per four consecutive output blocks;
-// a grid smaller than the texture's waves walks them grid-stride.
+// Index of this workgroup in a grid_2d launch (workgroup y * gridDim.x + x;
+// below 2^32 since y < 2^16 and x <= 2^16).
+__device__ __forceinline__ uint32_t group_index() { return blockIdx.y * gridDim.x + blockIdx.x; }
+
+// Plain per-texture launch: one wave per four consecutive output blocks.
 template <int FMT, int ALG>
 __global__ __launch_bounds__(64) void k_encode(const uint8_t* __restrict__ rgba, uint32_t width, uint32_t height,
                                                uint32_t blocks_wide, uint64_t nblocks, uint8_t* __restrict__ out,
@@ -29,18 +32,17 @@ __global__ __launch_bounds__(64) void k_encode(const uint8_t* __restrict__ rgba,
 {
     __shared__ GroupLds lds[4];
     const int tid = (int)threadIdx.x, g = tid >> 4, l = tid & 15;
-    const uint64_t nwaves = (nblocks + 3) / 4;
-    for (uint64_t wv = blockIdx.x; wv < nwaves; wv += gridDim.x) {
-        const uint64_t b = wv * 4u + (uint64_t)g;
-        const bool live = b < nblocks;
-        const uint32_t bx = live ? (uint32_t)(b % blocks_wide) : 0u;
-        const uint32_t by = live ? (uint32_t)(b / blocks_wide) : 0u;
-        bool valid;
-        const uint32_t px = fetch_pixel(rgba, width, height, bx, by, live, l, valid);
-        uint32_t w[4];
-        encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
-        if (live && l == 0) store_block<FMT>(out, b, w);
-    }
+    // the last row of a 2-D grid may hold groups past the texture: all their
+    // blocks are not live, as in a part-filled last wave
+    const uint64_t b = (uint64_t)group_index() * 4u + (uint64_t)g;
+    const bool live = b < nblocks;
+    const uint32_t bx = live ? (uint32_t)(b % blocks_wide) : 0u;
+    const uint32_t by = live ? (uint32_t)(b / blocks_wide) : 0u;
+    bool valid;
+    const uint32_t px = fetch_pixel(rgba, width, height, bx, by, live, l, valid);
+    uint32_t w[4];
+    encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
+    if (live && l == 0) store_block<FMT>(out, b, w);
 }
 
 // BC4 / BC5 per-texture launch: one block per lane (alpha codebooks are
@@ -51,15 +53,14 @@ __global__ __launch_bounds__(256) void k_encode_alpha(const uint8_t* __restrict_
                                                       uint32_t height, uint32_t blocks_wide, uint64_t nblocks,
                                                       uint8_t* __restrict__ out)
 {
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblocks;
-         b += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t px[16];
-        const uint32_t valid = fetch_block_lane(rgba, width, height, (uint32_t)(b % blocks_wide),
-                                                (uint32_t)(b / blocks_wide), px);
-        uint32_t w[4];
-        encode_alpha_lane<FMT>(px, valid, w);
-        store_block<FMT>(out, b, w);
-    }
+    const uint64_t b = (uint64_t)group_index() * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    uint32_t px[16];
+    const uint32_t valid = fetch_block_lane(rgba, width, height, (uint32_t)(b % blocks_wide),
+                                            (uint32_t)(b / blocks_wide), px);
+    uint32_t w[4];
+    encode_alpha_lane<FMT>(px, valid, w);
+    store_block<FMT>(out, b, w);
 }
 
 // RangeFit per-texture launch (BC1/BC2/BC3): one block per lane, the front of
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(256) void k_encode_alpha(const uint8_t* __restrict_
 // single-colour or range fit) without its record path.  A RangeFit block is a
 // few hundred serial operations; on a 16-lane group every fold would run on
 // all 16 lanes.  Single-wave workgroups (the per-wave LDS copies of the
-// block pixels and weights), grid-stride over the texture's 64-block spans.
+// block pixels and weights), one per 64-block span of the texture.
 template <int FMT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_encode_range_lane(
     const uint8_t* __restrict__ rgba, uint32_t width, uint32_t height, uint32_t blocks_wide, uint64_t nblocks,
@@ -76,20 +77,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ LaneTables LT;
     __shared__ LaneBlockLds LB;
     const int lane = (int)threadIdx.x;
+    const uint64_t base = (uint64_t)group_index() * 64u;
+    if (base >= nblocks) return;
     fill_lane_tables(LT, lane);
     group_sync();
-    for (uint64_t base = (uint64_t)blockIdx.x * 64u; base < nblocks; base += (uint64_t)gridDim.x * 64u) {
-        const uint64_t b = base + (uint64_t)lane;
-        if (b < nblocks) {
-            uint32_t px[16];
-            const uint32_t valid = fetch_block_lane(rgba, width, height, (uint32_t)(b % blocks_wide),
-                                                    (uint32_t)(b / blocks_wide), px);
-            uint32_t w[4];
-            LaneSet S;
-            float vx, vy, vz;
-            encode_front_lane<FMT, RANGE_FIT>(px, valid, fp, LT, LB, lane, w, S, vx, vy, vz);
-            store_block<FMT>(out, b, w);
-        }
+    const uint64_t b = base + (uint64_t)lane;
+    if (b < nblocks) {
+        uint32_t px[16];
+        const uint32_t valid = fetch_block_lane(rgba, width, height, (uint32_t)(b % blocks_wide),
+                                                (uint32_t)(b / blocks_wide), px);
+        uint32_t w[4];
+        LaneSet S;
+        float vx, vy, vz;
+        encode_front_lane<FMT, RANGE_FIT>(px, valid, fp, LT, LB, lane, w, S, vx, vy, vz);
+        store_block<FMT>(out, b, w);
     }
 }
 
@@ -102,17 +103,14 @@ __global__ __launch_bounds__(64) void k_encode_blocks(const uint8_t* __restrict_
 {
     __shared__ GroupLds lds[4];
     const int tid = (int)threadIdx.x, g = tid >> 4, l = tid & 15;
-    const uint64_t nwaves = (nblocks + 3) / 4;
-    for (uint64_t wv = blockIdx.x; wv < nwaves; wv += gridDim.x) {
-        const uint64_t b = wv * 4u + (uint64_t)g;
-        const bool live = b < nblocks;
-        const uint32_t mask = live ? masks[b] : 0u;
-        const bool valid = live && ((mask >> l) & 1u);
-        const uint32_t px = valid ? reinterpret_cast<const uint32_t*>(blocks + 64 * b)[l] : 0u;
-        uint32_t w[4];
-        encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
-        if (live && l == 0) store_block<FMT>(out, b, w);
-    }
+    const uint64_t b = (uint64_t)group_index() * 4u + (uint64_t)g;
+    const bool live = b < nblocks;
+    const uint32_t mask = live ? masks[b] : 0u;
+    const bool valid = live && ((mask >> l) & 1u);
+    const uint32_t px = valid ? reinterpret_cast<const uint32_t*>(blocks + 64 * b)[l] : 0u;
+    uint32_t w[4];
+    encode_block<FMT, ALG>(px, valid, fp, lds[g], g, l, w);
+    if (live && l == 0) store_block<FMT>(out, b, w);
 }
 
 // ------------------------------------------------------------------ decode
@@ -217,9 +215,9 @@ hipError_t launch_encode_t(const txk::EncodeArgs& a, hipStream_t s)
 {
     const uint64_t groups = (a.nblocks + 3) / 4;
     if (groups == 0) return hipSuccess;
-    hipLaunchKernelGGL((txd::k_encode<FMT, ALG>), dim3(txk::capped_grid(groups, txk::kMaxEncodeGroups)), dim3(64), 0,
-                       s, a.rgba, a.width, a.height, a.blocks_wide, a.nblocks, a.out,
-                       txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
+    if (groups > txk::kMaxGroups2d) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((txd::k_encode<FMT, ALG>), txk::grid_2d(groups), dim3(64), 0, s, a.rgba, a.width, a.height,
+                       a.blocks_wide, a.nblocks, a.out, txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
     return hipGetLastError();
 }
 
@@ -228,9 +226,9 @@ hipError_t launch_encode_range(const txk::EncodeArgs& a, hipStream_t s)
 {
     const uint64_t waves = (a.nblocks + 63) / 64;
     if (waves == 0) return hipSuccess;
-    hipLaunchKernelGGL((txd::k_encode_range_lane<FMT>), dim3(txk::capped_grid(waves, txk::kMaxEncodeGroups)), dim3(64),
-                       0, s, a.rgba, a.width, a.height, a.blocks_wide, a.nblocks, a.out,
-                       txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
+    if (waves > txk::kMaxGroups2d) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((txd::k_encode_range_lane<FMT>), txk::grid_2d(waves), dim3(64), 0, s, a.rgba, a.width, a.height,
+                       a.blocks_wide, a.nblocks, a.out, txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
     return hipGetLastError();
 }
 
@@ -249,9 +247,9 @@ hipError_t launch_blocks_t(const txk::BlockListArgs& a, hipStream_t s)
 {
     const uint64_t groups = (a.nblocks + 3) / 4;
     if (groups == 0) return hipSuccess;
-    hipLaunchKernelGGL((txd::k_encode_blocks<FMT, ALG>), dim3(txk::capped_grid(groups, txk::kMaxEncodeGroups)),
-                       dim3(64), 0, s, a.blocks, a.masks, a.nblocks, a.out,
-                       txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
+    if (groups > txk::kMaxGroups2d) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((txd::k_encode_blocks<FMT, ALG>), txk::grid_2d(groups), dim3(64), 0, s, a.blocks, a.masks,
+                       a.nblocks, a.out, txd::FitParams{a.wr, a.wg, a.wb, a.weigh_alpha});
     return hipGetLastError();
 }
 
@@ -285,8 +283,9 @@ hipError_t launch_encode_alpha(const EncodeArgs& a, hipStream_t s)
 {
     const uint64_t groups = (a.nblocks + 255) / 256;
     if (groups == 0) return hipSuccess;
-    hipLaunchKernelGGL((txd::k_encode_alpha<FMT>), dim3(capped_grid(groups, kMaxAlphaGroups)), dim3(256), 0, s, a.rgba,
-                       a.width, a.height, a.blocks_wide, a.nblocks, a.out);
+    if (groups > kMaxGroups2d) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((txd::k_encode_alpha<FMT>), grid_2d(groups), dim3(256), 0, s, a.rgba, a.width, a.height,
+                       a.blocks_wide, a.nblocks, a.out);
     return hipGetLastError();
 }
 

@@ -83,14 +83,25 @@ hipError_t launch_batch_light(const BatchTex* d_tex, const uint4* d_chunks, uint
 // Clears d_counter[0..1] on `s` (a kernel: replay-safe inside graphs).
 hipError_t reset_counters(uint32_t* d_counter, hipStream_t s);
 
-// Grid caps of the one-pass kernels: a launch never exceeds 2^32 work-items
-// in a dimension; larger inputs are walked grid-stride by the kernels.
-constexpr uint64_t kMaxEncodeGroups = 1ull << 22;   // 64-thread groups, 4 blocks each (16M blocks per pass)
-constexpr uint64_t kMaxAlphaGroups = 1ull << 20;    // 256-thread groups, one block per lane
+// Launch shapes that keep every grid dimension within its limit (2^32
+// work-items in x, 2^16 workgroups in y) whatever the texture size.  The
+// one-pass encoders take `groups` workgroups as a 2-D grid of rows of at
+// most 2^16 and index workgroup y * gridDim.x + x (a grid-stride loop
+// instead made the compiler keep more registers live and cost the lockstep
+// kernel a wave per SIMD); the decoder caps its tile grid and walks larger
+// images grid-stride.
+constexpr uint64_t kGroupsPerRow = 1ull << 16;
+constexpr uint64_t kMaxGroups2d = kGroupsPerRow * 65535ull;
 constexpr uint64_t kMaxDecodeTilesX = 1ull << 16;   // 64 x 4 pixel tiles per grid row
 constexpr uint64_t kMaxDecodeTilesY = 1ull << 14;
 
 inline uint32_t capped_grid(uint64_t groups, uint64_t cap) { return (uint32_t)(groups < cap ? groups : cap); }
+
+inline dim3 grid_2d(uint64_t groups)
+{
+    const uint64_t x = groups < kGroupsPerRow ? groups : kGroupsPerRow;
+    return dim3((uint32_t)x, (uint32_t)((groups + x - 1) / x));
+}
 
 hipError_t launch_encode(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s);
