@@ -158,11 +158,16 @@ def _download(bufs):
     return [d_out.download(n) for _, d_out, n in bufs]
 
 
-def bc1_cluster_gpu(tx, texgen, size, steps, warmup):
+def bc1_texture(texgen, size):
+    """The seeded size x size RGB texture of the BC1 cases (alpha 255)."""
+    return texgen.rgb_texture(size, size, texgen.SEED_BASE)
+
+
+def bc1_cluster_gpu(tx, texgen, size, steps, warmup, img=None):
     """BC1 ClusterFit on the seeded size x size RGB texture (alpha 255,
     default perceptual weights), device-resident: the timed GPU half of a
     bc1 case (512 is the CPU-runnable configuration)."""
-    img = texgen.rgb_texture(size, size, texgen.SEED_BASE)
+    img = bc1_texture(texgen, size) if img is None else img
     p = tx.Params()
     d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
     n = tx.Format.Bc1.compressed_size(size, size)
@@ -277,14 +282,15 @@ CHAIN_CASES = [("BC1", 0, "RangeFit"), ("BC1", 0, "ClusterFit"), ("BC2", 1, "Clu
                ("BC3", 2, "IterativeClusterFit"), ("BC4", 3, "ClusterFit"), ("BC1", 0, "IterativeClusterFit")]
 
 
-def suite_chains(tx, texgen, O, threads, steps, warmup, tuning_path):
+def suite_chains(tx, texgen, O, threads, steps, warmup, tuning_path, retune=True):
     """Full mip chain of a seeded 4096x4096 texture, per format;
-    the chain is one batch with the autotuned tuning for that format."""
+    the chain is one batch with the autotuned tuning for that format
+    (re-tuned first unless `retune` is false)."""
     chain = chain_4096(texgen)
     rows = []
     for name, fmt, alg in CHAIN_CASES:
         rows.append(chain_case(tx, O, chain, name, fmt, tx.Algorithm[alg], threads, steps, warmup, tuning_path,
-                               retune=True))
+                               retune=retune))
         print("suite: " + json.dumps(rows[-1]), file=sys.stderr, flush=True)
     return rows
 
@@ -492,7 +498,8 @@ def run_suite(args):
         out.append(suite_bc1_512(tx, texgen, O, threads, args.steps, args.warmup))
         print(json.dumps(out[-1]), flush=True)
     if "chains" in which:
-        for r in suite_chains(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table):
+        for r in suite_chains(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table,
+                              retune=not args.keep_tuning):
             out.append(r)
             print(json.dumps(r), flush=True)
     if "batch" in which:
@@ -525,6 +532,8 @@ def main():
                                                  "default line")
     ap.add_argument("--suite-out", default="", help="write the suite results to this JSON file")
     ap.add_argument("--tuning-table", default=os.path.join(ROOT, "texpresso_amd", "tuning.json"))
+    ap.add_argument("--keep-tuning", action="store_true",
+                    help="chains suite: use the tunings the table holds instead of re-running the autotuner")
     ap.add_argument("--batch-count", type=int, default=1000)
     ap.add_argument("--qps", default="100,250,500,1000,2000", help="serving arrival-rate sweep (jobs/s)")
     ap.add_argument("--pipeline-threshold", type=int, default=-1,
@@ -565,27 +574,11 @@ def main():
     if tx.lib().tx_device_arch() is None:
         raise SystemExit("no gfx950 device usable by libtexpresso_hip.so")
 
-    # GPU phases first, CPU phases (oracle parity, CPU baselines) last, so the
-    # GPU runs its timed work back to back without idling between them
-    pending = []
-    chain = None
-    if not args.no_cpu:
-        # secondaries: BC1 ClusterFit (the metric's other format) at 512^2 and
-        # 2048^2, and the 4096^2 mip chain in BC1 and BC3
-        # IterativeClusterFit, batched and per level
-        for size in (512, 2048):
-            pending.append(("bc1_clusterfit_%d" % size, "bc1", bc1_cluster_gpu(tx, texgen, size, args.steps,
-                                                                             args.warmup)))
-            print("bench: BC1 ClusterFit %d^2 %.4f ms/step" % (size, pending[-1][2]["ms"]), file=sys.stderr,
-                  flush=True)
-        chain = chain_4096(texgen)
-        for name, f, alg in (("BC1", 0, "IterativeClusterFit"), ("BC3", 2, "IterativeClusterFit")):
-            g = chain_gpu(tx, chain, name, f, tx.Algorithm[alg], args.steps, args.warmup, args.tuning_table,
-                          retune=False)
-            pending.append(("chain4096_%s_%s" % (name.lower(), alg.lower()), "chain", g))
-            print("bench: 4096^2 chain %s %s batched %.3f ms, per level %.3f ms" %
-                  (name, alg, g["ms_batch"], g["ms_plain"]), file=sys.stderr, flush=True)
-
+    # inputs first (generated on the CPU and uploaded), then the GPU phases
+    # back to back, then the CPU phases (oracle parity, CPU baselines): the
+    # GPU's clocks drop while it idles (the split search runs ~1.3 ms per
+    # launch for its first launches after an idle stretch, ~1.11 ms once
+    # they settle), so no CPU work sits between the timed GPU phases
     W = H = args.size
     seed = texgen.SEED_BASE + 1
     img = texgen.with_alpha_field(texgen.rgb_texture(W, H, seed), seed)
@@ -595,6 +588,31 @@ def main():
     out_len = fmt.compressed_size(W, H)
     d_in = tx.DeviceBuffer.from_array(img.reshape(-1))
     d_out = tx.DeviceBuffer(out_len)
+    n5 = tx.Format.Bc5.compressed_size(W, H)
+    d_nm = tx.DeviceBuffer.from_array(nmap.reshape(-1))
+    d_nm_out = tx.DeviceBuffer(n5)
+    pending = []
+    chain = None
+    if not args.no_cpu:
+        # secondaries: BC1 ClusterFit (the metric's other format) at 512^2 and
+        # 2048^2, and the 4096^2 mip chain in BC1 and BC3
+        # IterativeClusterFit, batched and per level
+        bc1_imgs = {size: bc1_texture(texgen, size) for size in (512, 2048)}
+        chain = chain_4096(texgen)
+        for size in (512, 2048):
+            pending.append(("bc1_clusterfit_%d" % size, "bc1", bc1_cluster_gpu(tx, texgen, size, args.steps,
+                                                                             args.warmup, bc1_imgs[size])))
+            print("bench: BC1 ClusterFit %d^2 %.4f ms/step" % (size, pending[-1][2]["ms"]), file=sys.stderr,
+                  flush=True)
+        for name, f, alg in (("BC1", 0, "IterativeClusterFit"), ("BC3", 2, "IterativeClusterFit")):
+            g = chain_gpu(tx, chain, name, f, tx.Algorithm[alg], args.steps, args.warmup, args.tuning_table,
+                          retune=False)
+            pending.append(("chain4096_%s_%s" % (name.lower(), alg.lower()), "chain", g))
+            print("bench: 4096^2 chain %s %s batched %.3f ms, per level %.3f ms" %
+                  (name, alg, g["ms_batch"], g["ms_plain"]), file=sys.stderr, flush=True)
+
+    # BC5 on the normal map (second half of the workload), same protocol
+    ms5, _ = time_device(tx, tx.Format.Bc5, d_nm, W, H, tx.Params(), d_nm_out, n5, args.steps, args.warmup)
     print("bench: BC3 IterativeClusterFit %dx%d, %d warmup + %d steps" % (W, H, args.warmup, args.steps),
           file=sys.stderr, flush=True)
     ms, wall = time_device(tx, fmt, d_in, W, H, params, d_out, out_len, args.steps, args.warmup)
@@ -607,11 +625,6 @@ def main():
     # its own, events around its launches and a wait for it to finish
     lat_dev = [tx.time_compress(fmt, d_in.ptr, W, H, params, d_out.ptr, out_len, 1) for _ in range(max(50, args.steps))]
 
-    # BC5 on the normal map (second half of the workload), same protocol
-    n5 = tx.Format.Bc5.compressed_size(W, H)
-    d_nm = tx.DeviceBuffer.from_array(nmap.reshape(-1))
-    d_nm_out = tx.DeviceBuffer(n5)
-    ms5, _ = time_device(tx, tx.Format.Bc5, d_nm, W, H, tx.Params(), d_nm_out, n5, args.steps, args.warmup)
     print("bench: BC3 %.3f ms/step (%.1f Mpix/s), BC5 %.3f ms/step" % (ms, value, ms5), file=sys.stderr, flush=True)
 
     line = {
