@@ -209,3 +209,31 @@ def test_repeat_runs_identical_and_tiling_independent(gpu):
     blocks = img.reshape(h // 4, 4, w // 4, 4, 4).transpose(0, 2, 1, 3, 4).reshape(-1, 64)
     c = tx.compress_blocks(2, blocks, np.full(len(blocks), 0xFFFF, np.uint32), p)
     assert np.array_equal(a, c.reshape(-1))
+
+
+def adversarial_texture(seed, bw=131, bh=101, cut=(3, 2)):
+    """The adversarial block set tiled row-major into a texture of bw x bh
+    blocks (13,231 blocks, repeating the set), cut by `cut` pixels on the
+    right and bottom so the last column and row are partial blocks."""
+    blocks, _ = make_blocks(seed, per_class=128)
+    reps = -(-(bw * bh) // len(blocks))
+    tiled = np.tile(blocks, (reps, 1))[: bw * bh]
+    img = tiled.reshape(bh, bw, 4, 4, 4).transpose(0, 2, 1, 3, 4).reshape(4 * bh, 4 * bw, 4)
+    return np.ascontiguousarray(img[: 4 * bh - cut[1], : 4 * bw - cut[0]])
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2], ids=["bc1", "bc2", "bc3"])
+@pytest.mark.parametrize("wi", range(len(WEIGHTS)), ids=["perceptual", "uniform", "skewed"])
+@pytest.mark.parametrize("alpha", [False, True], ids=["plain", "alphaw"])
+def test_range_fit_lane_kernel_adversarial_texture(gpu, oracle, fmt, wi, alpha):
+    # RangeFit textures of at least 12,288 blocks take the one-block-per-lane
+    # kernel: every adversarial class (single colours, NaN axes, ties,
+    # transparency, alpha extremes, small palettes, noise) in one texture,
+    # with partial blocks in its last row and column
+    img = adversarial_texture(3000 + 11 * fmt + wi)
+    h, w = img.shape[:2]
+    assert tx.num_blocks(w) * tx.num_blocks(h) >= 12288
+    p = _params(tx.Algorithm.RangeFit, WEIGHTS[wi], alpha)
+    got = gpu_compress(fmt, img, w, h, p)
+    want = oracle.compress(fmt, img, w, h, _oparams(oracle, p), threads=oracle.default_threads())
+    assert np.array_equal(got, want), report(fmt, got, want, oracle)

@@ -232,11 +232,19 @@ hipError_t launch_encode_range(const txk::EncodeArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// RangeFit textures of at least this many blocks take the one-block-per-lane
+// kernel, smaller ones the 16-lane-group kernel: a lane-per-block wave costs
+// one block's serial latency (~17 us on MI355X however few blocks it has),
+// while the group kernel spreads a block over 16 lanes (measured per level
+// of a 4096^2 chain, BC1: 11.6 vs 17.1 us at 4096 blocks, 18.5 vs 16.8 us
+// at 16384, 166 vs 33 us at 262144).
+constexpr uint64_t kRangeLaneMinBlocks = 12288;
+
 template <int FMT>
 hipError_t launch_encode_f(const txk::EncodeArgs& a, hipStream_t s)
 {
     switch (a.algorithm) {
-    case 0: return launch_encode_range<FMT>(a, s);
+    case 0: return a.nblocks >= kRangeLaneMinBlocks ? launch_encode_range<FMT>(a, s) : launch_encode_t<FMT, 0>(a, s);
     case 1: return launch_encode_t<FMT, 1>(a, s);
     default: return launch_encode_t<FMT, 2>(a, s);
     }

@@ -68,6 +68,39 @@ __device__ __forceinline__ uint32_t colour_key(uint32_t px, uint32_t inc, int p)
     return ((inc >> p) & 1u) ? (px & 0xFFFFFFu) : 0x1000000u + (uint32_t)p;
 }
 
+// Bit m set iff key == keys[m], m = 0..15.  Each compare writes VCC and one
+// add-with-carry shifts the mask left and brings the compare in at bit 0
+// (mask + mask + carry), keys visited from 15 down to 0: two instructions
+// per key, where compare, select and or take three and more.  The carry
+// hand-off is the one of a 64-bit add (v_add_co / v_addc_co), which needs no
+// wait states.  One asm block, so nothing is scheduled between the pairs.
+__device__ __forceinline__ uint32_t match_mask16(uint32_t key, const uint32_t (&keys)[16])
+{
+    uint32_t mask = 0u;
+    asm("v_cmp_eq_u32_e32 vcc, %1, %17\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %16\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %15\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %14\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %13\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %12\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %11\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %10\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %9\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %8\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %7\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %6\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %5\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %4\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %3\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+        "v_cmp_eq_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+        : "+v"(mask)
+        : "v"(key), "v"(keys[0]), "v"(keys[1]), "v"(keys[2]), "v"(keys[3]), "v"(keys[4]), "v"(keys[5]),
+          "v"(keys[6]), "v"(keys[7]), "v"(keys[8]), "v"(keys[9]), "v"(keys[10]), "v"(keys[11]), "v"(keys[12]),
+          "v"(keys[13]), "v"(keys[14]), "v"(keys[15])
+        : "vcc");
+    return mask;
+}
+
 // colourset.rs:35-112 on one lane.  For each pixel p, same = the pixels with
 // p's colour (p included): p opens a point when none of them is lower; its
 // point index counts the points opened below its lowest same-colour pixel;
@@ -92,19 +125,26 @@ __device__ __forceinline__ void colour_set_lane(const uint32_t (&px)[16], uint32
     S.transparent = FMT == BC1 && transp != 0;
     uint32_t first = 0;
     uint64_t remap = 0;
+    uint32_t keys[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) keys[m] = colour_key(px[m], inc, m);
     // the pixel loop stays rolled (pixel p comes from LDS) and the 16
     // compares against the register copies are unrolled
 #pragma unroll 1
     for (int p = 0; p < 16; ++p) {
         const uint32_t kp = colour_key(L.px[p][lane], inc, p);
-        // e = 1 when the keys match: keys are below 2^25, so for distinct
-        // keys (difference - 1) never has bit 31 set
         uint32_t same = 0, asum = 0;
+        if (fp.weigh_alpha) {
+            // e = 1 when the keys match: keys are below 2^25, so for
+            // distinct keys (difference - 1) never has bit 31 set
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const uint32_t e = ((kp ^ colour_key(px[m], inc, m)) - 1u) >> 31;
-            same |= e << m;
-            if (fp.weigh_alpha) asum += e * ((px[m] >> 24) + 1u);
+            for (int m = 0; m < 16; ++m) {
+                const uint32_t e = ((kp ^ keys[m]) - 1u) >> 31;
+                same |= e << m;
+                asum += e * ((px[m] >> 24) + 1u);
+            }
+        } else {
+            same = match_mask16(kp, keys);
         }
         const bool in = (inc >> p) & 1u;
         const bool opens = in && (same & ((1u << p) - 1u)) == 0u;
