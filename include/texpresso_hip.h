@@ -134,7 +134,7 @@ int tx_decompress_device(int format, const uint8_t* d_data, size_t width, size_t
  * output.  Calls on one stream are ordered by the stream.  The array keeps
  * the last call's contents; every call clears the counters it needs. */
 #define TX_NOALLOC_MAX_PIPELINE_BLOCKS 262144u /* a 2048 x 2048 texture */
-#define TX_NOALLOC_SCRATCH_BYTES 92275200u     /* 88 MiB: 352 bytes per block plus the counters */
+#define TX_NOALLOC_SCRATCH_BYTES 92301056u     /* 88 MiB: 352 bytes per block plus the counters */
 int tx_compress_device_noalloc(int format, const uint8_t* d_rgba, size_t width, size_t height,
                                const tx_params* params, uint8_t* d_output, size_t out_len, void* stream);
 

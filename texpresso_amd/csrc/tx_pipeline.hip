@@ -52,15 +52,42 @@ __device__ __forceinline__ const BatchTex& tex_at(const BatchTex* tex, const Bat
     return tex ? tex[i] : tex0;
 }
 
-// Per-workgroup bin counts of the classify pass, flushed once at its end.
-struct ClassifyHist {
-    uint32_t n[kLists][17];
-};
+// Calls f(g, m) once for every value g of `group` among the lanes with `in`
+// set, m = the ballot of those lanes.  Uniform over the wave.
+template <class F>
+__device__ __forceinline__ void for_each_group(bool in, uint32_t group, F&& f)
+{
+    uint64_t rem = __ballot(in);
+    while (rem) {
+        const int src = (int)__builtin_ctzll(rem);
+        const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)group, src);
+        const uint64_t m = __ballot(in && group == g);
+        rem &= ~m;
+        f(g, m);
+    }
+}
 
-// ClusterFit block on one lane: its record into slot r, its key, and its
-// count in the workgroup's bins.
+// Counts a wave of keys (lane slot `slot`) into the count copies: one ballot
+// per (copy, colour count) present among the lanes of each list the texture
+// variant feeds, and one atomic add by its first lane.
 template <int FMT, int ALG>
-__device__ __forceinline__ uint32_t write_record_lane(ClusterRec* __restrict__ recs, uint32_t r, ClassifyHist& H,
+__device__ __forceinline__ void count_keys(PipeCtl* __restrict__ ctl, uint32_t key, uint32_t slot, int lane)
+{
+    const uint32_t group = ((slot / txp::kBinSeg) % txp::kHistCopies) << 5 | (key & 31u);
+#pragma unroll
+    for (int k = 0; k < kLists; ++k) {
+        if (ALG == RANGE_FIT) continue;
+        if (ALG == CLUSTER_FIT && k != kListC) continue;
+        if (ALG == ITERATIVE_CLUSTER_FIT && (k == kListC || (k == kListA && FMT != BC1))) continue;
+        for_each_group((key & key_bit(k)) != 0u, group, [&](uint32_t g, uint64_t m) {
+            if (lane == (int)__builtin_ctzll(m)) atomicAdd(&ctl->part[g >> 5][k][g & 31u], (uint32_t)__popcll(m));
+        });
+    }
+}
+
+// ClusterFit block on one lane: its record into slot r, and its key.
+template <int FMT, int ALG>
+__device__ __forceinline__ uint32_t write_record_lane(ClusterRec* __restrict__ recs, uint32_t r,
                                                       const LaneSet& S, const LaneTables& T, const uint32_t (&out)[4],
                                                       float vx, float vy, float vz, const FitParams& fp,
                                                       uint32_t tex_idx, uint32_t block)
@@ -93,22 +120,21 @@ __device__ __forceinline__ uint32_t write_record_lane(ClusterRec* __restrict__ r
         if (FMT == BC1) key |= key_bit(kListA);
         if (need4) key |= key_bit(kListB);
     }
-    for (int k = 0; k < kLists; ++k)
-        if (key & key_bit(k)) atomicAdd(&H.n[k][S.n], 1u);
     return key;
 }
 
 // Blocks [first, first + count) of one texture, one per lane, 64 at a time.
 // Single-colour and range-fit blocks are finished here; ClusterFit blocks
-// become records in slots [slot0, slot0 + count).
+// become records in slots [slot0, slot0 + count), counted in ctl->part.
 template <int FMT, int ALG>
 __device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_idx, uint32_t first, uint32_t count,
                                                uint32_t slot0, const LaneTables& LT, LaneBlockLds& LB, int lane,
-                                               ClusterRec* recs, uint32_t* keys, ClassifyHist& H)
+                                               ClusterRec* recs, uint32_t* keys, PipeCtl* ctl)
 {
     const FitParams fp{T.wr, T.wg, T.wb, T.weigh_alpha};
     for (uint32_t off = 0; off < count; off += 64) {
         const uint32_t k = off + (uint32_t)lane;
+        uint32_t key = 0;
         if (k < count) {
             const uint32_t b = first + k;
             uint32_t px[16];
@@ -118,9 +144,8 @@ __device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_i
             LaneSet S;
             float vx, vy, vz;
             const int kind = encode_front_lane<FMT, ALG>(px, valid, fp, LT, LB, lane, w, S, vx, vy, vz);
-            uint32_t key = 0;
             if (kind == KIND_CLUSTER) {
-                key = write_record_lane<FMT, ALG>(recs, slot0 + k, H, S, LT, w, vx, vy, vz, fp, tex_idx, b);
+                key = write_record_lane<FMT, ALG>(recs, slot0 + k, S, LT, w, vx, vy, vz, fp, tex_idx, b);
             } else if (FMT == BC1) {
                 *reinterpret_cast<uint2*>(T.out + (size_t)b * 8u) = make_uint2(w[0], w[1]);
             } else {
@@ -128,6 +153,7 @@ __device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_i
             }
             keys[slot0 + k] = key;
         }
+        count_keys<FMT, ALG>(ctl, key, slot0 + k, lane);
     }
 }
 
@@ -142,11 +168,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 {
     __shared__ LaneTables LT;
     __shared__ LaneBlockLds LB;
-    __shared__ ClassifyHist H;
     const int lane = (int)threadIdx.x;
     fill_lane_tables(LT, lane);
-    if (lane < 17)
-        for (int k = 0; k < kLists; ++k) H.n[k][lane] = 0u;
     group_sync();
     // chunks hold similar work, so a plain grid-stride loop balances them
     // (a shared work counter would serialise thousands of atomics at launch)
@@ -160,58 +183,67 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         const BatchTex& T = tex_at(tex, tex0, ch.x);
         switch (T.variant) {
-        case 0: classify_chunk<BC1, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        case 1: classify_chunk<BC1, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        case 2: classify_chunk<BC1, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        case 3: classify_chunk<BC2, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        case 4: classify_chunk<BC2, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        case 5: classify_chunk<BC2, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        case 6: classify_chunk<BC3, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        case 7: classify_chunk<BC3, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        default: classify_chunk<BC3, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, H); break;
-        }
-    }
-    // one flush of the workgroup's bin counts
-    group_sync();
-    if (lane < 17) {
-        for (int k = 0; k < kLists; ++k) {
-            const uint32_t c = H.n[k][lane];
-            if (c) atomicAdd(&ctl->hist[k][lane], c);
+        case 0: classify_chunk<BC1, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
+        case 1: classify_chunk<BC1, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
+        case 2: classify_chunk<BC1, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
+        case 3: classify_chunk<BC2, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
+        case 4: classify_chunk<BC2, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
+        case 5: classify_chunk<BC2, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
+        case 6: classify_chunk<BC3, RANGE_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
+        case 7: classify_chunk<BC3, CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
+        default: classify_chunk<BC3, ITERATIVE_CLUSTER_FIT>(T, ch.x, ch.y, ch.z, ch.w, LT, LB, lane, recs, keys, ctl); break;
         }
     }
 }
 
-// Counting sort of the records by colour count, 16 first.  Each workgroup
-// takes kBinSeg consecutive slots: it counts its keys per bin in LDS,
-// reserves one range per non-empty bin with a single global atomic, then
-// places its records inside those ranges (order within a bin is free: it
-// only steers which records share a wave).
-constexpr uint32_t kBinSeg = 1024;
-
+// Counting sort of the records by colour count, 16 first, one workgroup
+// per slot segment s (kBinSeg slots).  Bin (list k, colour count v) starts
+// after bins v+1..16 of its list and holds the records of count copy 0, then
+// copy 1, ...; the workgroup sums the copies into the totals (workgroup 0
+// also stores them in ctl->hist for the search passes) and into the start
+// of its own copy c = s % kHistCopies, counts its keys per bin in LDS,
+// reserves one range per non-empty bin with one atomic add on copy c's
+// cursor, then places its records inside those ranges (order within a bin
+// is free: it only steers which records share a wave).
 __global__ __launch_bounds__(256) void k_pipe_bin(const uint32_t* __restrict__ keys, uint64_t nslots,
                                                   PipeCtl* __restrict__ ctl, uint32_t* __restrict__ list_a,
                                                   uint32_t* __restrict__ list_b, uint32_t* __restrict__ list_c)
 {
-    __shared__ uint32_t cnt[kLists][17], base[kLists][17];
+    constexpr uint32_t kCounts = kLists * 17;
+    __shared__ uint32_t cnt[kLists][17], base[kLists][17], tot[kLists][17], pre[kLists][17];
     uint32_t* const lists[kLists] = {list_a, list_b, list_c};
     const uint32_t t = threadIdx.x;
+    const uint32_t copy = blockIdx.x % txp::kHistCopies;
     if (t < 17)
-        for (int k = 0; k < kLists; ++k) cnt[k][t] = 0u;
+        for (int k = 0; k < kLists; ++k) cnt[k][t] = tot[k][t] = pre[k][t] = 0u;
     __syncthreads();
-    const uint64_t s0 = (uint64_t)blockIdx.x * kBinSeg;
-    const uint64_t s1 = s0 + kBinSeg < nslots ? s0 + kBinSeg : nslots;
+    // totals and the start of this copy: 4 threads per count, each summing
+    // every 4th copy
+    if (t < 4 * kCounts) {
+        const uint32_t km = t % kCounts;
+        uint32_t sum = 0, below = 0;
+        for (uint32_t c = t / kCounts; c < txp::kHistCopies; c += 4) {
+            const uint32_t v = (&ctl->part[c][0][0])[km];
+            sum += v;
+            below += c < copy ? v : 0u;
+        }
+        if (sum) atomicAdd(&(&tot[0][0])[km], sum);
+        if (below) atomicAdd(&(&pre[0][0])[km], below);
+    }
+    const uint64_t s0 = (uint64_t)blockIdx.x * txp::kBinSeg;
+    const uint64_t s1 = s0 + txp::kBinSeg < nslots ? s0 + txp::kBinSeg : nslots;
     for (uint64_t r = s0 + t; r < s1; r += blockDim.x) {
         const uint32_t key = keys[r];
         for (int k = 0; k < kLists; ++k)
             if (key & key_bit(k)) atomicAdd(&cnt[k][key & 31u], 1u);
     }
     __syncthreads();
+    if (blockIdx.x == 0 && t < kCounts) (&ctl->hist[0][0])[t] = (&tot[0][0])[t];
     if (t < 17) {
-        // bins are laid out 16 first: bin n starts after the counts of bins n+1..16
         for (int k = 0; k < kLists; ++k) {
-            uint32_t o = 0;
-            for (int m = 16; m > (int)t; --m) o += ctl->hist[k][m];
-            base[k][t] = cnt[k][t] ? o + atomicAdd(&ctl->pos[k][t], cnt[k][t]) : 0u;
+            uint32_t o = pre[k][t];
+            for (int m = 16; m > (int)t; --m) o += tot[k][m];
+            base[k][t] = cnt[k][t] ? o + atomicAdd(&ctl->pos[copy][k][t], cnt[k][t]) : 0u;
             cnt[k][t] = 0u;
         }
     }
@@ -473,7 +505,7 @@ hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s
 {
     if (p.nslots > b.capacity || p.nslots > 0xFFFFFFFFull) return hipErrorInvalidValue;
     auto mark = [&](int i) { return marks ? hipEventRecord(marks[i], s) : hipSuccess; };
-    hipLaunchKernelGGL(txd::k_pipe_reset, dim3(1), dim3(64), 0, s, b.ctl);
+    hipLaunchKernelGGL(txd::k_pipe_reset, dim3(1), dim3(256), 0, s, b.ctl);
     if (p.nchunks == 0 || p.nslots == 0) {
         for (int i = 0; i < 5; ++i)
             if (mark(i) != hipSuccess) return hipErrorUnknown;
@@ -483,7 +515,7 @@ hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s
     hipLaunchKernelGGL(txd::k_pipe_classify, dim3(p.classify_grid), dim3(64), 0, s, p.d_tex, p.tex0, p.d_chunks,
                        p.nchunks, p.chunk_blocks, b.recs, b.keys, b.ctl);
     if (mark(1) != hipSuccess) return hipErrorUnknown;
-    const uint32_t bin_grid = (uint32_t)((p.nslots + txd::kBinSeg - 1) / txd::kBinSeg);
+    const uint32_t bin_grid = (uint32_t)((p.nslots + kBinSeg - 1) / kBinSeg);
     hipLaunchKernelGGL(txd::k_pipe_bin, dim3(bin_grid), dim3(256), 0, s, b.keys, p.nslots, b.ctl, b.list[kListA],
                        b.list[kListB], b.list[kListC]);
     if (mark(2) != hipSuccess) return hipErrorUnknown;

@@ -54,10 +54,24 @@ static_assert(offsetof(ClusterRec, remap) == 256 && offsetof(ClusterRec, alpha) 
 // takes one global atomic per block.
 enum { kListA = 0, kListB = 1, kListC = 2, kLists = 3 };
 
+// The bin pass sorts kBinSeg consecutive record slots per workgroup.
+constexpr uint32_t kBinSeg = 1024;
+
+// Bin counts and bin cursors are kept in kHistCopies copies: the records of
+// slot segment s (slots [s * kBinSeg, (s + 1) * kBinSeg)) are counted in copy
+// s % kHistCopies, and bin (list k, colour count v) is laid out copy after
+// copy.  Device-scope atomic adds on one word serialise; with a single copy,
+// thousands of classify workgroups finishing together, nearly all counting
+// 16-colour blocks, queued on the same word (2048x2048: ~27 us per list of
+// a ~70 us pass), and every bin workgroup on the same cursor.  With copies,
+// no word takes more than a few dozen adds.
+constexpr uint32_t kHistCopies = 64;
+
 struct PipeCtl {
     uint32_t cursor[kLists];
-    uint32_t hist[kLists][17];  // records per colour count
-    uint32_t pos[kLists][17];   // records placed per colour count (bin pass)
+    uint32_t hist[kLists][17];                // records per colour count (totals, written by the bin pass)
+    uint32_t part[kHistCopies][kLists][17];   // records per colour count, copy by copy (classify pass)
+    uint32_t pos[kHistCopies][kLists][17];    // records placed per colour count, copy by copy (bin pass)
 };
 
 struct PipeBufs {
