@@ -642,64 +642,76 @@ __device__ __forceinline__ bool ordering_step(GroupLds& L, int l, int n, bool ac
     return unique;
 }
 
+// Largest value of v over the four 16-lane groups of the wave (v uniform
+// within each group), as a scalar.
+__device__ __forceinline__ uint32_t wave_max_of_groups(uint32_t v)
+{
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+    const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return max(max(a, b), max(c, d));
+}
+
 // Candidate loop of one split search (see search_step).
 template <int NC, bool UNIT>
 __device__ __forceinline__ void search_loop(const GroupLds& L, int l, uint32_t mylen, const float4& xs,
                                             const FitParams& fp, float& be, uint32_t& bc)
 {
-    // The code list is read through a buffer resource: one byte offset,
-    // advanced once per trip, with the +16/+32/+48 entries as immediate
-    // offsets (and hardware bounds checks: a read past the end returns 0).
-    // The 16-byte unit entries are read from a uniform base plus a 32-bit
-    // byte offset.
+    // Trip q covers list entries 32q .. 32q + 31: lane l evaluates entry
+    // 32q + l (A) and 32q + 16 + l (B), each while 32q is below its limit.
+    // Loop control and the trip's offsets are scalar: the code list is read
+    // through a buffer resource with the lane's offset in a vector register
+    // and the trip's in a scalar one (hardware bounds checks: a read past
+    // the end returns 0), the 16-byte unit entries from a scalar base plus
+    // the lane's offset.  Codes are read two half-trips ahead and prefix
+    // sums one half-trip ahead.  The kept minimum is the smallest (error,
+    // code), codes ordered by loop-order rank, so evaluation order does not
+    // matter; the three comparisons are combined without branches.  The
+    // lists are padded past the longest trip: reads need no guard, and
+    // entries at or past `mylen` are loaded but never evaluated.
     constexpr int kStored = NC == 4 ? txt::kCand4Stored : txt::kCand3Stored;
     const __amdgpu_buffer_rsrc_t rt =
         __builtin_amdgcn_make_buffer_rsrc((void*)(NC == 4 ? c_cand.t4 : c_cand.t3), 0, kStored * 4, kBufferFlags);
     const char* ubase = reinterpret_cast<const char*>(NC == 4 ? c_unit.u4 : c_unit.u3);
-    auto code_at = [&](uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b32(rt, off, 0, 0); };
-    auto unit_at = [&](uint32_t off) { return *reinterpret_cast<const txt::UnitSplit*>(ubase + off); };
+    const uint32_t vo = 4u * (uint32_t)l, uo = 16u * (uint32_t)l;
+    auto code_at = [&](uint32_t soff) { return __builtin_amdgcn_raw_buffer_load_b32(rt, vo, soff, 0); };
+    auto unit_at = [&](const char* base) { return *reinterpret_cast<const txt::UnitSplit*>(base + uo); };
+    const uint32_t limA = mylen > (uint32_t)l ? mylen - (uint32_t)l : 0u;
+    const uint32_t limB = mylen > (uint32_t)l + 16u ? mylen - (uint32_t)l - 16u : 0u;
+    const uint32_t wlen = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_max_of_groups(mylen));
     be = __builtin_huge_valf();
     bc = 0xFFFFFFFFu;
-    // Two candidates per lane and trip (A at t, B at t + 16).  Codes are read
-    // two half-trips ahead and prefix sums one half-trip ahead; writing each
-    // carried value back in the trip that consumed it keeps it in the same
-    // registers (no copies at the back edge).  The kept minimum is the
-    // smallest (error, code), so evaluation order does not matter.
-    // the lists are padded past the longest trip: reads need no guard, and
-    // entries at or past `mylen` are loaded but never evaluated
-    uint32_t ot = 4u * (uint32_t)l, ou = 16u * (uint32_t)l;  // byte offsets of entry t
-    uint32_t cA = code_at(ot);
-    uint32_t cB = code_at(ot + 64);
+    auto keep = [&](float err, uint32_t c) {
+        const bool take = (err < be) | ((err == be) & (c < bc));
+        be = take ? err : be;
+        bc = take ? c : bc;
+    };
+    uint32_t cA = code_at(0);
+    uint32_t cB = code_at(64);
     SplitSums sA = load_split<NC>(L, cA);
     txt::UnitSplit uA{};
-    if (UNIT) uA = unit_at(ou);
-    for (uint32_t base = 0; wave_any(base < mylen); base += 32) {
-        const uint32_t t = base + (uint32_t)l;
-        const uint32_t cA2 = code_at(ot + 128);
+    if (UNIT) uA = unit_at(ubase);
+    for (uint32_t base = 0; base < wlen; base += 32) {
+        const uint32_t so = 4u * base;
+        const char* ub = ubase + 16u * base;
+        const uint32_t cA2 = code_at(so + 128);
         const SplitSums sB = load_split<NC>(L, cB);
         txt::UnitSplit uB{};
-        if (UNIT) uB = unit_at(ou + 256);
-        if (t < mylen) {
+        if (UNIT) uB = unit_at(ub + 256);
+        if (base < limA) {
             float a[3], b[3];
-            const float err = eval_split<NC, UNIT>(sA, xs, fp, a, b, uA);
-            const bool better = err < be || (err == be && cA < bc);
-            be = better ? err : be;
-            bc = better ? cA : bc;
+            keep(eval_split<NC, UNIT>(sA, xs, fp, a, b, uA), cA);
         }
-        const uint32_t cB2 = code_at(ot + 192);
+        const uint32_t cB2 = code_at(so + 192);
         sA = load_split<NC>(L, cA2);
-        if (UNIT) uA = unit_at(ou + 512);
-        if (t + 16 < mylen) {
+        if (UNIT) uA = unit_at(ub + 512);
+        if (base < limB) {
             float a[3], b[3];
-            const float err = eval_split<NC, UNIT>(sB, xs, fp, a, b, uB);
-            const bool better = err < be || (err == be && cB < bc);
-            be = better ? err : be;
-            bc = better ? cB : bc;
+            keep(eval_split<NC, UNIT>(sB, xs, fp, a, b, uB), cB);
         }
         cA = cA2;
         cB = cB2;
-        ot += 128;
-        ou += 512;
     }
 }
 
