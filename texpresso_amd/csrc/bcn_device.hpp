@@ -495,23 +495,33 @@ __device__ __forceinline__ float eval_split(const SplitSums& s, const float4& xs
         const float p3x = ((xs.x - p2.x) - p1.x) - p0.x;
         const float p3y = ((xs.y - p2.y) - p1.y) - p0.y;
         const float p3z = ((xs.z - p2.z) - p1.z) - p0.z;
-        const float t13 = 1.0f / 3.0f, t23 = 2.0f / 3.0f, n19 = 1.0f / 9.0f, n49 = 4.0f / 9.0f;
         // alphax = part2*(1/3,1/3,1/3,1/9) + (part1*(2/3,2/3,2/3,4/9) + part0)
-        ax[0] = p2.x * t13 + (p1.x * t23 + p0.x);
-        ax[1] = p2.y * t13 + (p1.y * t23 + p0.y);
-        ax[2] = p2.z * t13 + (p1.z * t23 + p0.z);
-        // betax = part1*(1/3,..,1/9) + (part2*(2/3,..,4/9) + part3)
-        bx[0] = p1.x * t13 + (p2.x * t23 + p3x);
-        bx[1] = p1.y * t13 + (p2.y * t23 + p3y);
-        bx[2] = p1.z * t13 + (p2.z * t23 + p3z);
+        // betax  = part1*(1/3,..,1/9) + (part2*(2/3,..,4/9) + part3)
+        // The binary32 constants satisfy 2/3 = 2 * (1/3) and 4/9 = 4 * (1/9)
+        // exactly (same significand), and scaling by a power of two commutes
+        // with rounding for the normal values summed here, so
+        // p * (2/3) + q == fma(2, p * (1/3), q) bit for bit: one product per
+        // part serves both alphax and betax.
+        const float t13 = 1.0f / 3.0f, n19 = 1.0f / 9.0f;
+        static_assert(2.0f * (1.0f / 3.0f) == 2.0f / 3.0f && 4.0f * (1.0f / 9.0f) == 4.0f / 9.0f,
+                      "2/3 and 4/9 are exact multiples of 1/3 and 1/9");
+        const float m1x = p1.x * t13, m1y = p1.y * t13, m1z = p1.z * t13;
+        const float m2x = p2.x * t13, m2y = p2.y * t13, m2z = p2.z * t13;
+        ax[0] = m2x + __builtin_fmaf(2.0f, m1x, p0.x);
+        ax[1] = m2y + __builtin_fmaf(2.0f, m1y, p0.y);
+        ax[2] = m2z + __builtin_fmaf(2.0f, m1z, p0.z);
+        bx[0] = m1x + __builtin_fmaf(2.0f, m2x, p3x);
+        bx[1] = m1y + __builtin_fmaf(2.0f, m2y, p3y);
+        bx[2] = m1z + __builtin_fmaf(2.0f, m2z, p3z);
         if (UNIT) {
             a2 = u.a2;
             b2 = u.b2;
             ab = u.ab;
         } else {
             const float p3w = ((xs.w - p2.w) - p1.w) - p0.w;
-            a2 = p2.w * n19 + (p1.w * n49 + p0.w);
-            b2 = p1.w * n19 + (p2.w * n49 + p3w);
+            const float w1 = p1.w * n19, w2 = p2.w * n19;
+            a2 = w2 + __builtin_fmaf(4.0f, w1, p0.w);
+            b2 = w1 + __builtin_fmaf(4.0f, w2, p3w);
             ab = (2.0f / 9.0f) * (p1.w + p2.w);
         }
     } else {
