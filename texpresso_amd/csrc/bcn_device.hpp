@@ -413,51 +413,69 @@ __device__ __forceinline__ void range_fit(const LDS& L, const ColourSet& cs, int
     s0 = snap(s0, 31.0f, 1.0f / 31.0f); s1 = snap(s1, 63.0f, 1.0f / 63.0f); s2 = snap(s2, 31.0f, 1.0f / 31.0f);
     e0 = snap(e0, 31.0f, 1.0f / 31.0f); e1 = snap(e1, 63.0f, 1.0f / 63.0f); e2 = snap(e2, 31.0f, 1.0f / 31.0f);
 
-    float best = FLT_MAX;
-    for (int pass = (FMT == BC1 ? 3 : 4); pass <= 4; ++pass) {
-        if (pass == 4 && FMT == BC1 && cs.transparent) break;
-        float cx[4], cy[4], cz[4];
-        int nc;
-        cx[0] = s0; cy[0] = s1; cz[0] = s2;
-        cx[1] = e0; cy[1] = e1; cz[1] = e2;
-        if (pass == 3) {
-            cx[2] = s0 * 0.5f + e0 * 0.5f; cy[2] = s1 * 0.5f + e1 * 0.5f; cz[2] = s2 * 0.5f + e2 * 0.5f;
-            cx[3] = 0.0f; cy[3] = 0.0f; cz[3] = 0.0f;
-            nc = 3;
-        } else {
-            const float t2 = 2.0f / 3.0f, t1 = 1.0f / 3.0f;
-            cx[2] = s0 * t2 + e0 * t1; cy[2] = s1 * t2 + e1 * t1; cz[2] = s2 * t2 + e2 * t1;
-            cx[3] = s0 * t1 + e0 * t2; cy[3] = s1 * t1 + e1 * t2; cz[3] = s2 * t1 + e2 * t2;
-            nc = 4;
+    // range.rs:104-141: closest code per point, error folded in point order,
+    // for the 3-colour pass (BC1) and the 4-colour pass (unless BC1 with
+    // transparent pixels) in one loop: both codebooks start with the snapped
+    // start and end, so a point's distances to those two serve both passes.
+    const bool do3 = FMT == BC1;
+    const bool do4 = !(FMT == BC1 && cs.transparent);
+    const float t2 = 2.0f / 3.0f, t1 = 1.0f / 3.0f;
+    const float h0 = s0 * 0.5f + e0 * 0.5f, h1 = s1 * 0.5f + e1 * 0.5f, h2 = s2 * 0.5f + e2 * 0.5f;
+    const float f0 = s0 * t2 + e0 * t1, f1 = s1 * t2 + e1 * t1, f2 = s2 * t2 + e2 * t1;
+    const float g0 = s0 * t1 + e0 * t2, g1 = s1 * t1 + e1 * t2, g2 = s2 * t1 + e2 * t2;
+    auto dist_to = [&](const float4& q, float cx, float cy, float cz) {
+        const float dx = fp.wr * (q.x - cx);
+        const float dy = fp.wg * (q.y - cy);
+        const float dz = fp.wb * (q.z - cz);
+        return dx * dx + dy * dy + dz * dz;
+    };
+    float error3 = 0.0f, error4 = 0.0f;
+    uint32_t mine3 = 0, mine4 = 0;
+    for (int p = 0; p < n; ++p) {
+        const float4 q = L.pts[p];
+        const float d0 = dist_to(q, s0, s1, s2);
+        const float d1 = dist_to(q, e0, e1, e2);
+        float m01 = FLT_MAX;  // first strict minimum of codes 0 and 1
+        uint32_t k01 = 0;
+        if (d0 < m01) m01 = d0;
+        if (d1 < m01) {
+            m01 = d1;
+            k01 = 1;
         }
-        // range.rs:104-141: closest code per point, error folded in point order
-        float error = 0.0f;
-        uint32_t mine = 0;
-        for (int p = 0; p < n; ++p) {
-            const float4 q = L.pts[p];
-            float dist = FLT_MAX;
-            uint32_t idx = 0;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (j < nc) {
-                    const float dx = fp.wr * (q.x - cx[j]);
-                    const float dy = fp.wg * (q.y - cy[j]);
-                    const float dz = fp.wb * (q.z - cz[j]);
-                    const float d = dx * dx + dy * dy + dz * dz;
-                    if (d < dist) {
-                        dist = d;
-                        idx = (uint32_t)j;
-                    }
-                }
+        if (do3) {
+            float dist = m01;
+            uint32_t k = k01;
+            const float d2 = dist_to(q, h0, h1, h2);
+            if (d2 < dist) {
+                dist = d2;
+                k = 2;
             }
-            if (p == cs.remap) mine = idx;
-            error += dist;
+            if (p == cs.remap) mine3 = k;
+            error3 += dist;
         }
-        if (error < best) {
-            write_colour(pass == 4, s0, s1, s2, e0, e1, e2, cs.remap < 0 ? 3u : mine, l, lo, hi);
-            best = error;
+        if (do4) {
+            float dist = m01;
+            uint32_t k = k01;
+            const float d2 = dist_to(q, f0, f1, f2);
+            if (d2 < dist) {
+                dist = d2;
+                k = 2;
+            }
+            const float d3 = dist_to(q, g0, g1, g2);
+            if (d3 < dist) {
+                dist = d3;
+                k = 3;
+            }
+            if (p == cs.remap) mine4 = k;
+            error4 += dist;
         }
     }
+    float best = FLT_MAX;
+    if (do3 && error3 < best) {
+        write_colour(false, s0, s1, s2, e0, e1, e2, cs.remap < 0 ? 3u : mine3, l, lo, hi);
+        best = error3;
+    }
+    if (do4 && error4 < best) write_colour(true, s0, s1, s2, e0, e1, e2, cs.remap < 0 ? 3u : mine4, l, lo, hi);
 }
 
 // The three prefix sums a split reads: part0 = S[0][i], part1, part2.

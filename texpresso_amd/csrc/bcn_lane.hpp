@@ -54,6 +54,9 @@ struct LaneSet {
 };
 
 // Colour of a pixel as a point (x, y, z in [0, 1]), from the u8 / 255 table.
+// (An arithmetic form, c * (1/255) corrected by one fma step from the exact
+// residual, is correctly rounded for all 256 bytes but measured no faster:
+// its fma and conversion issue at half the rate of plain adds.)
 __device__ __forceinline__ void xyz(const LaneTables& T, uint32_t rgb, float& x, float& y, float& z)
 {
     x = T.u8f[rgb & 255u];
@@ -313,53 +316,76 @@ __device__ __forceinline__ void range_fit_lane(const LaneSet& S, const LaneTable
     s0 = snap(s0, 31.0f, 1.0f / 31.0f); s1 = snap(s1, 63.0f, 1.0f / 63.0f); s2 = snap(s2, 31.0f, 1.0f / 31.0f);
     e0 = snap(e0, 31.0f, 1.0f / 31.0f); e1 = snap(e1, 63.0f, 1.0f / 63.0f); e2 = snap(e2, 31.0f, 1.0f / 31.0f);
 
-    float best = FLT_MAX;
-    for (int pass = (FMT == BC1 ? 3 : 4); pass <= 4; ++pass) {
-        if (pass == 4 && FMT == BC1 && S.transparent) break;
-        float cx[4], cy[4], cz[4];
-        int nc;
-        cx[0] = s0; cy[0] = s1; cz[0] = s2;
-        cx[1] = e0; cy[1] = e1; cz[1] = e2;
-        if (pass == 3) {
-            cx[2] = s0 * 0.5f + e0 * 0.5f; cy[2] = s1 * 0.5f + e1 * 0.5f; cz[2] = s2 * 0.5f + e2 * 0.5f;
-            cx[3] = 0.0f; cy[3] = 0.0f; cz[3] = 0.0f;
-            nc = 3;
-        } else {
-            const float t2 = 2.0f / 3.0f, t1 = 1.0f / 3.0f;
-            cx[2] = s0 * t2 + e0 * t1; cy[2] = s1 * t2 + e1 * t1; cz[2] = s2 * t2 + e2 * t1;
-            cx[3] = s0 * t1 + e0 * t2; cy[3] = s1 * t1 + e1 * t2; cz[3] = s2 * t1 + e2 * t2;
-            nc = 4;
-        }
-        // range.rs:104-141: closest code per point, error folded in point order
-        float error = 0.0f;
-        uint32_t idx = 0;
+    // range.rs:104-141 for the 3-colour pass (BC1) and the 4-colour pass
+    // (unless BC1 with transparent pixels) in one loop over the pixels: the
+    // two codebooks share codes 0 and 1 (the snapped start and end), so each
+    // pixel's distances to them are computed once and serve both passes.
+    // Each pass keeps its own first strict minimum per pixel (codes in
+    // index order) and its own error folded in point order, exactly as two
+    // separate passes would.
+    const bool do3 = FMT == BC1;
+    const bool do4 = !(FMT == BC1 && S.transparent);
+    const float t2 = 2.0f / 3.0f, t1 = 1.0f / 3.0f;
+    const float h0 = s0 * 0.5f + e0 * 0.5f, h1 = s1 * 0.5f + e1 * 0.5f, h2 = s2 * 0.5f + e2 * 0.5f;
+    const float f0 = s0 * t2 + e0 * t1, f1 = s1 * t2 + e1 * t1, f2 = s2 * t2 + e2 * t1;
+    const float g0 = s0 * t1 + e0 * t2, g1 = s1 * t1 + e1 * t2, g2 = s2 * t1 + e2 * t2;
+    auto dist_to = [&](float x, float y, float z, float cx, float cy, float cz) {
+        const float dx = fp.wr * (x - cx);
+        const float dy = fp.wg * (y - cy);
+        const float dz = fp.wb * (z - cz);
+        return dx * dx + dy * dy + dz * dz;
+    };
+    float error3 = 0.0f, error4 = 0.0f;
+    uint32_t idx3 = 0, idx4 = 0;
 #pragma unroll 2
-        for (int p = 0; p < 16; ++p) {
-            float x, y, z;
-            xyz(T, S.pix(p), x, y, z);
-            float dist = FLT_MAX;
-            uint32_t k = 0;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (j < nc) {
-                    const float dx = fp.wr * (x - cx[j]);
-                    const float dy = fp.wg * (y - cy[j]);
-                    const float dz = fp.wb * (z - cz[j]);
-                    const float d = dx * dx + dy * dy + dz * dz;
-                    if (d < dist) {
-                        dist = d;
-                        k = (uint32_t)j;
-                    }
-                }
-            }
-            idx |= (((S.inc >> p) & 1u) ? k : 3u) << (2 * p);
-            if ((S.first >> p) & 1u) error += dist;
+    for (int p = 0; p < 16; ++p) {
+        float x, y, z;
+        xyz(T, S.pix(p), x, y, z);
+        const float d0 = dist_to(x, y, z, s0, s1, s2);
+        const float d1 = dist_to(x, y, z, e0, e1, e2);
+        // codes 0 and 1: the first strict minimum starts from FLT_MAX
+        float m01 = FLT_MAX;
+        uint32_t k01 = 0;
+        if (d0 < m01) m01 = d0;
+        if (d1 < m01) {
+            m01 = d1;
+            k01 = 1;
         }
-        if (error < best) {
-            write_colour_lane(pass == 4, s0, s1, s2, e0, e1, e2, idx, lo, hi);
-            best = error;
+        const bool in = (S.inc >> p) & 1u, pt = (S.first >> p) & 1u;
+        if (do3) {
+            float dist = m01;
+            uint32_t k = k01;
+            const float d2 = dist_to(x, y, z, h0, h1, h2);
+            if (d2 < dist) {
+                dist = d2;
+                k = 2;
+            }
+            idx3 |= (in ? k : 3u) << (2 * p);
+            if (pt) error3 += dist;
+        }
+        if (do4) {
+            float dist = m01;
+            uint32_t k = k01;
+            const float d2 = dist_to(x, y, z, f0, f1, f2);
+            if (d2 < dist) {
+                dist = d2;
+                k = 2;
+            }
+            const float d3 = dist_to(x, y, z, g0, g1, g2);
+            if (d3 < dist) {
+                dist = d3;
+                k = 3;
+            }
+            idx4 |= (in ? k : 3u) << (2 * p);
+            if (pt) error4 += dist;
         }
     }
+    float best = FLT_MAX;
+    if (do3 && error3 < best) {
+        write_colour_lane(false, s0, s1, s2, e0, e1, e2, idx3, lo, hi);
+        best = error3;
+    }
+    if (do4 && error4 < best) write_colour_lane(true, s0, s1, s2, e0, e1, e2, idx4, lo, hi);
 }
 
 // BC2 alpha of one lane's block (alpha.rs:27-51, same steps as bc2_alpha).
