@@ -130,6 +130,27 @@ def valu_probe(waves_per_simd):
     return rate.value if L.txb_valu_probe(waves_per_simd, ctypes.byref(rate)) == 0 else None
 
 
+def op_rates():
+    """Rows {mix, wave_instrs_per_us_per_simd, lanes_per_clk_at_2400mhz} from
+    the probe library's instruction-mix kernels (8 waves per SIMD)."""
+    import ctypes
+    path = os.path.join(ROOT, "texpresso_amd", "libtx_bench_probe.so")
+    if not os.path.exists(path):
+        sys.exit("libtx_bench_probe.so is not built")
+    L = ctypes.CDLL(path)
+    L.txb_op_mix_name.restype = ctypes.c_char_p
+    L.txb_op_mix_name.argtypes = [ctypes.c_int]
+    L.txb_op_rate.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    rows = []
+    for m in range(L.txb_op_mixes()):
+        rate = ctypes.c_double()
+        if L.txb_op_rate(m, ctypes.byref(rate)) != 0:
+            sys.exit("op-rate probe failed on mix %d" % m)
+        rows.append({"mix": L.txb_op_mix_name(m).decode(), "wave_instrs_per_us_per_simd": round(rate.value / 1e6, 1),
+                     "lanes_per_clk_at_2400mhz": round(rate.value * 64 / 2.4e9, 2)})
+    return rows
+
+
 def time_device(tx, fmt, d_in, w, h, params, d_out, out_len, steps, warmup):
     """Mean kernel milliseconds per step from HIP events recorded on the
     library stream around `steps` launches, plus the wall time around them."""
@@ -524,6 +545,8 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
+    ap.add_argument("--op-rates", action="store_true",
+                    help="issue rate of single instruction forms and two-form mixes (benchmark probe library)")
     ap.add_argument("--valu-sweep", action="store_true",
                     help="print the plain FP32 VALU issue rate at 1..8 waves per SIMD and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -547,6 +570,10 @@ def main():
         if args.gpus != 1:
             sys.exit(2)
         run_suite(args)
+        return
+    if args.op_rates:
+        for row in op_rates():
+            print(json.dumps(row), flush=True)
         return
     if args.valu_sweep:
         for w in range(1, 9):

@@ -8,6 +8,10 @@
 // The operands are per-lane values in vector registers, as in the encoder's
 // split loop: an operand in a scalar register (a kernel argument used
 // directly) halves the issue rate of these instructions on this GPU.
+//
+// A second probe (txb_op_rate) times fixed instruction mixes -- one
+// instruction form, or two forms interleaved -- to show which forms issue
+// at the plain FP32 rate and which can issue alongside one another.
 #include <hip/hip_runtime.h>
 
 #pragma clang fp contract(off)
@@ -40,7 +44,102 @@ __global__ __launch_bounds__(256) void k_valu_probe(float* out, int iters, float
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// Op-rate mixes: eight independent chains per lane, each step one
+// instruction written in inline assembly (register operands only, no memory
+// access) so that the compiler neither folds nor reorders the chains.
+#define TXB_OP(ins, x) asm volatile(ins : "+v"(x) : "v"(b), "v"(c))
+#define TXB_MIX2(i0, i1)                                                                                   \
+    TXB_OP(i0, x0); TXB_OP(i1, x1); TXB_OP(i0, x2); TXB_OP(i1, x3); TXB_OP(i0, x4); TXB_OP(i1, x5);        \
+    TXB_OP(i0, x6); TXB_OP(i1, x7)
+
+constexpr const char* kOpMixNames[] = {
+    "8 v_add_f32",          "8 v_mul_f32",          "8 v_fma_f32",         "8 v_add_u32",
+    "8 v_sad_u16",          "8 v_min3_u32",         "8 v_trunc_f32",       "8 v_bfe_u32",
+    "8 v_mul_u32_u24",      "4 v_sad_u16 + 4 v_add_f32", "4 v_trunc_f32 + 4 v_mul_f32",
+    "4 v_min3_u32 + 4 v_add_u32"};
+constexpr int kOpMixes = (int)(sizeof(kOpMixNames) / sizeof(kOpMixNames[0]));
+
+template <int M>
+__global__ __launch_bounds__(256) void k_op_rate(uint32_t* out, int iters, uint32_t seed)
+{
+    const uint32_t t = threadIdx.x + blockIdx.x * 256u;
+    uint32_t x0 = t ^ seed, x1 = t + 1u, x2 = t * 3u, x3 = t + 7u, x4 = t ^ 0x55u, x5 = t + 11u, x6 = t * 5u,
+             x7 = t + 13u;
+    const uint32_t b = (seed | 1u) + (t & 1u), c = (seed >> 3) + (t & 2u);
+    for (int i = 0; i < iters; ++i) {
+        if (M == 0) { TXB_MIX2("v_add_f32 %0, %0, %1", "v_add_f32 %0, %0, %1"); }
+        if (M == 1) { TXB_MIX2("v_mul_f32 %0, %0, %1", "v_mul_f32 %0, %0, %1"); }
+        if (M == 2) { TXB_MIX2("v_fma_f32 %0, %0, %1, %2", "v_fma_f32 %0, %0, %1, %2"); }
+        if (M == 3) { TXB_MIX2("v_add_u32 %0, %0, %1", "v_add_u32 %0, %0, %1"); }
+        if (M == 4) { TXB_MIX2("v_sad_u16 %0, %0, %1, %2", "v_sad_u16 %0, %0, %1, %2"); }
+        if (M == 5) { TXB_MIX2("v_min3_u32 %0, %0, %1, %2", "v_min3_u32 %0, %0, %1, %2"); }
+        if (M == 6) { TXB_MIX2("v_trunc_f32 %0, %0", "v_trunc_f32 %0, %0"); }
+        if (M == 7) { TXB_MIX2("v_bfe_u32 %0, %0, %1, %2", "v_bfe_u32 %0, %0, %1, %2"); }
+        if (M == 8) { TXB_MIX2("v_mul_u32_u24 %0, %0, %1", "v_mul_u32_u24 %0, %0, %1"); }
+        if (M == 9) { TXB_MIX2("v_sad_u16 %0, %0, %1, %2", "v_add_f32 %0, %0, %1"); }
+        if (M == 10) { TXB_MIX2("v_trunc_f32 %0, %0", "v_mul_f32 %0, %0, %1"); }
+        if (M == 11) { TXB_MIX2("v_min3_u32 %0, %0, %1, %2", "v_add_u32 %0, %0, %1"); }
+    }
+    out[t] = x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7;
+}
+#undef TXB_MIX2
+#undef TXB_OP
+
+template <int M>
+void launch_op_rate(int blocks, uint32_t* out, int iters, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_op_rate<M>, dim3(blocks), dim3(256), 0, s, out, iters, 7u);
+}
+
+typedef void (*OpLaunch)(int, uint32_t*, int, hipStream_t);
+constexpr OpLaunch kOpLaunch[kOpMixes] = {
+    launch_op_rate<0>, launch_op_rate<1>, launch_op_rate<2>, launch_op_rate<3>, launch_op_rate<4>,  launch_op_rate<5>,
+    launch_op_rate<6>, launch_op_rate<7>, launch_op_rate<8>, launch_op_rate<9>, launch_op_rate<10>, launch_op_rate<11>};
+
 }  // namespace
+
+// Number of instruction mixes of txb_op_rate and the name of mix m (nullptr
+// when m is out of range).
+extern "C" int txb_op_mixes(void) { return kOpMixes; }
+extern "C" const char* txb_op_mix_name(int m) { return m >= 0 && m < kOpMixes ? kOpMixNames[m] : nullptr; }
+
+// Mix m at 8 waves per SIMD: VALU wave-instructions per second per SIMD
+// (the loop's few scalar instructions aside).  Returns 0 on success, -1 on a
+// bad argument, -2 on a HIP error.
+extern "C" int txb_op_rate(int m, double* wave_instrs_per_s_per_simd)
+{
+    if (!wave_instrs_per_s_per_simd || m < 0 || m >= kOpMixes) return -1;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return -2;
+    const int blocks = prop.multiProcessorCount * 8, iters = 4096;
+    uint32_t* out = nullptr;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return -2;
+    if (hipMalloc(&out, sizeof(uint32_t) * blocks * 256) != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return -2;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) ? 0 : -2;
+    if (rc == 0) {
+        kOpLaunch[m](blocks, out, iters, s);  // warms the clocks
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < 4; ++r) kOpLaunch[m](blocks, out, iters, s);
+        (void)hipEventRecord(e1, s);
+        rc = hipEventSynchronize(e1) == hipSuccess ? 0 : -2;
+    }
+    float ms = 0.0f;
+    if (rc == 0 && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = -2;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    (void)hipStreamDestroy(s);
+    if (rc != 0 || ms <= 0.0f) return rc != 0 ? rc : -2;
+    const double instrs = 4.0 * blocks * 4.0 /* waves per block */ * iters * 8.0;
+    *wave_instrs_per_s_per_simd = instrs / (ms * 1e-3) / (prop.multiProcessorCount * 4.0);
+    return 0;
+}
 
 // Plain FP32 lane-operations per second on the current device with
 // `waves_per_simd` (1..8) waves per SIMD.  Returns 0 on success, -1 on a bad
