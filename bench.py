@@ -251,10 +251,20 @@ def chain_gpu(tx, chain, name, fmt, alg, steps, warmup, tuning_path, retune):
     tx.time_per_texture(descs, max(2, warmup // 4))
     ms_plain = tx.time_per_texture(descs, steps) / steps
     got_plain = _download(bufs)
+    # the same per-level launches with the binned pipeline off: every level
+    # through the one-pass lockstep kernel (the plainest launch there is)
+    old = tx.set_pipeline_threshold(2 ** 64 - 1)
+    try:
+        tx.time_per_texture(descs, 2)
+        ms_onepass = tx.time_per_texture(descs, max(3, steps // 4)) / max(3, steps // 4)
+        got_onepass = _download(bufs)
+    finally:
+        tx.set_pipeline_threshold(old)
     for d_in, d_out, _ in bufs:
         d_in.free()
         d_out.free()
     return {"name": name, "fmt": fmt, "alg": alg, "weights": p.weights, "got": got, "got_plain": got_plain,
+            "got_onepass": got_onepass, "ms_onepass": ms_onepass,
             "ms_batch": ms_batch, "ms_plain": ms_plain, "tuning": best.as_dict(),
             "block_size": tx.Format(fmt).block_size()}
 
@@ -285,9 +295,12 @@ def chain_cpu(O, chain, g, threads):
             "batch_ms": round(ms_batch, 4), "batch_mpix_s": round(px / 1e6 / (ms_batch / 1e3), 1),
             "per_level_ms": round(ms_plain, 4), "per_level_mpix_s": round(px / 1e6 / (ms_plain / 1e3), 1),
             "batch_over_per_level": round(ms_plain / ms_batch, 3),
+            "per_level_onepass_ms": round(g["ms_onepass"], 4),
+            "batch_over_per_level_onepass": round(g["ms_onepass"] / ms_batch, 3),
             "cpu_mpix_s": round(px / 1e6 / t_cpu, 3), "cpu_threads": threads,
             "gpu_over_cpu": round((px / ms_batch / 1e3) / (px / t_cpu / 1e6), 1),
             "mismatched_blocks": bad_blocks(g["got"]), "mismatched_blocks_per_level": bad_blocks(g["got_plain"]),
+            "mismatched_blocks_per_level_onepass": bad_blocks(g["got_onepass"]),
             "tuning": g["tuning"],
             "valu_tflops": round(flops / (ms_batch / 1e3) / 1e12, 3),
             "valu_frac_of_157": round(flops / (ms_batch / 1e3) / 1e12 / VALU_PEAK_TFLOPS, 4),
