@@ -112,6 +112,10 @@ struct Launch {
     size_t chunk_cap = 0;
     uint8_t* pipe_mem = nullptr;     // binned-pipeline records of this slot (grown on demand)
     txp::PipeBufs pipe;
+    // chunk tables outgrown while serving, freed when the server is destroyed:
+    // hipFree and hipHostFree wait for the whole device, which would stall
+    // the launches of the other slots
+    std::vector<void*> retired_host, retired_dev;
     std::vector<uint64_t> jobs;
     bool busy = false;
 };
@@ -169,8 +173,8 @@ bool grow_chunks(Launch& L, size_t need)
 {
     if (need <= L.chunk_cap) return true;
     size_t cap = std::max<size_t>(need, L.chunk_cap * 2);
-    if (L.h_chunks) (void)hipHostFree(L.h_chunks);
-    if (L.d_chunks) (void)hipFree(L.d_chunks);
+    if (L.h_chunks) L.retired_host.push_back(L.h_chunks);
+    if (L.d_chunks) L.retired_dev.push_back(L.d_chunks);
     L.h_chunks = nullptr;
     L.d_chunks = nullptr;
     L.chunk_cap = 0;
@@ -462,7 +466,12 @@ int tx_server_create(tx_server** out, const tx_server_config* config)
         ok = ok && hipHostMalloc(&L.h_tex, sizeof(txk::BatchTex) * cfg.max_jobs_per_launch) == hipSuccess;
         ok = ok && hipMalloc(&L.d_tex, sizeof(txk::BatchTex) * cfg.max_jobs_per_launch) == hipSuccess;
         ok = ok && hipMalloc(&L.d_counter, 256) == hipSuccess;
-        ok = ok && grow_chunks(L, 4096);
+        // chunk table for a launch of 2^20 colour blocks (one 4096^2 texture)
+        // at the tuned chunk size, or one chunk per resident wave for small
+        // launches, without growing while serving
+        ok = ok && grow_chunks(L, (size_t)s->grid * cfg.tuning.waves_per_group +
+                                      ((size_t)1 << 20) / cfg.tuning.chunk_blocks +
+                                      2 * (size_t)cfg.max_jobs_per_launch);
         // record scratch for the largest launch the arena can hold (its
         // inputs take 64 bytes per block), allocated once: a later free would
         // wait for the whole device in the middle of serving
@@ -630,6 +639,8 @@ int tx_server_destroy(tx_server* s)
         if (L.d_chunks) (void)hipFree(L.d_chunks);
         if (L.d_counter) (void)hipFree(L.d_counter);
         if (L.pipe_mem) (void)hipFree(L.pipe_mem);
+        for (void* p : L.retired_host) (void)hipHostFree(p);
+        for (void* p : L.retired_dev) (void)hipFree(p);
     }
     if (s->fetch_stream) (void)hipStreamDestroy(s->fetch_stream);
     s->arena.release();
