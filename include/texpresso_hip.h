@@ -270,6 +270,18 @@ int tx_util_upload(void* d_dst, const void* src, size_t bytes);
 int tx_util_download(void* dst, const void* d_src, size_t bytes);
 /* Device-to-device copy on the library stream (synchronous). */
 int tx_util_copy(void* d_dst, const void* d_src, size_t bytes);
+/* The encoder's exactly specified elementary operations as the device
+ * evaluates them, for an exhaustive check against a host IEEE binary32
+ * evaluation: u8f[u] = u / 255 (the point coordinates of colourset.rs:60-66),
+ * sqrt_counts[c] = sqrtf(c) (point weights of c duplicate pixels),
+ * sqrt_alpha[k-1] = sqrtf(k / 256) for k = 1..4096 (alpha-weighted point
+ * weights, colourset.rs:103-110), and for each of the n values in[i] (n <=
+ * 2^26): snap31[i] / snap63[i] = the 5-/6-bit grid snap of range.rs and
+ * cluster.rs (clamp to [0, 1], truncf(g * v + 0.5) * (1/g)) and round255[i] =
+ * roundf(in[i]) clamped to [0, 255], NaN to 0 (math.rs:100-102).  Synchronous;
+ * host pointers. */
+int tx_util_elementary(const float* in, size_t n, float u8f[256], float sqrt_counts[17], float sqrt_alpha[4096],
+                       float* snap31, float* snap63, int32_t* round255);
 /* Enqueues `iters` tx_compress_device launches between two events on the
  * library stream, waits, and stores the elapsed milliseconds. */
 int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,

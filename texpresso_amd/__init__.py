@@ -171,6 +171,7 @@ def lib():
             "tx_util_upload": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_download": (c_int, [c_vp, c_vp, c_sz]),
             "tx_util_copy": (c_int, [c_vp, c_vp, c_sz]),
+            "tx_util_elementary": (c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
             "tx_set_pipeline_threshold": (ctypes.c_uint64, [ctypes.c_uint64]),
             "tx_util_time_compress": (c_int, [c_int, c_vp, c_sz, c_sz, pp, c_vp, c_sz, c_int,
                                               ctypes.POINTER(ctypes.c_float)]),
@@ -415,6 +416,27 @@ def set_pipeline_threshold(min_blocks):
     """ClusterFit textures with at least `min_blocks` blocks use the binned
     pipeline (0: always, 2**64-1: never).  Returns the previous value."""
     return lib().tx_set_pipeline_threshold(int(min_blocks))
+
+
+def elementary(values):
+    """The device's evaluation of the encoder's exactly specified operations
+    (tx_util_elementary): a dict with the u8/255 table, sqrtf of the counts
+    0..16 and of k/256 for k = 1..4096, and for each float32 in `values` the
+    5-/6-bit grid snaps and roundf clamped to [0, 255]."""
+    import numpy as np
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    n = int(v.size)
+    u8f = np.zeros(256, np.float32)
+    sqc = np.zeros(17, np.float32)
+    sqa = np.zeros(4096, np.float32)
+    s31 = np.zeros(max(n, 1), np.float32)
+    s63 = np.zeros(max(n, 1), np.float32)
+    r255 = np.zeros(max(n, 1), np.int32)
+    _check(lib().tx_util_elementary(v.ctypes.data if n else None, n, u8f.ctypes.data, sqc.ctypes.data,
+                                    sqa.ctypes.data, s31.ctypes.data, s63.ctypes.data, r255.ctypes.data),
+           "elementary")
+    return {"u8f": u8f, "sqrt_counts": sqc, "sqrt_alpha": sqa, "snap31": s31[:n], "snap63": s63[:n],
+            "round255": r255[:n]}
 
 
 def time_compress(fmt, d_rgba, width, height, params, d_out, out_len, iters):

@@ -1,6 +1,8 @@
 // bcn_kernels.hip -- kernels and launchers of the BC1-BC5 codec (gfx950).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bcn_device.hpp"
 #include "bcn_kernels.hpp"
 #include "bcn_lane.hpp"
@@ -204,6 +206,35 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ data
     }
 }
 
+// The exactly specified elementary operations of the encoder, evaluated by
+// the device functions the kernels use, for an exhaustive check against a
+// host IEEE binary32 evaluation: the u8/255 and sqrt(count) tables the
+// one-block-per-lane front fills (fill_lane_tables), sqrtf(k / 256) of the
+// alpha-weighted point weights (k = 1..4096, the range of the summed
+// (alpha + 1) values), and for each input value the 5- and 6-bit grid snap
+// (snap, range.rs / cluster.rs) and roundf clamped to [0, 255]
+// (f32_to_i32_clamped, math.rs:100-102).  tables: 256 + 17 + 4096 floats.
+__global__ __launch_bounds__(64) void k_elementary(const float* __restrict__ in, uint32_t n, float* __restrict__ tables,
+                                                   float* __restrict__ snap31, float* __restrict__ snap63,
+                                                   int32_t* __restrict__ round255)
+{
+    __shared__ LaneTables T;
+    const int lane = (int)threadIdx.x;
+    if (blockIdx.x == 0) {
+        fill_lane_tables(T, lane);
+        group_sync();
+        for (int i = lane; i < 256; i += 64) tables[i] = T.u8f[i];
+        if (lane < 17) tables[256 + lane] = T.sqc[lane];
+        for (int k = lane + 1; k <= 4096; k += 64) tables[256 + 17 + (k - 1)] = sqrtf((float)k / 256.0f);
+    }
+    for (uint32_t i = blockIdx.x * 64u + (uint32_t)lane; i < n; i += gridDim.x * 64u) {
+        const float v = in[i];
+        snap31[i] = snap(v, 31.0f, 1.0f / 31.0f);
+        snap63[i] = snap(v, 63.0f, 1.0f / 63.0f);
+        round255[i] = f32_to_i32_clamped(v, 255);
+    }
+}
+
 }  // namespace txd
 
 // ---------------------------------------------------------------- launchers
@@ -319,6 +350,15 @@ hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s)
     case 4: return launch_blocks_t<4, 0>(a, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_elementary(const float* d_in, uint32_t n, float* d_tables, float* d_snap31, float* d_snap63,
+                             int32_t* d_round255, hipStream_t s)
+{
+    const uint32_t groups = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, ((uint64_t)n + 63) / 64));
+    hipLaunchKernelGGL(txd::k_elementary, dim3(groups), dim3(64), 0, s, d_in, n, d_tables, d_snap31, d_snap63,
+                       d_round255);
+    return hipGetLastError();
 }
 
 hipError_t launch_decode(int format, const uint8_t* data, uint32_t w, uint32_t h, uint8_t* out, hipStream_t s)

@@ -106,6 +106,10 @@ inline dim3 grid_2d(uint64_t groups)
 hipError_t launch_encode(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_encode_blocks(const BlockListArgs& a, hipStream_t s);
 hipError_t launch_decode(int format, const uint8_t* data, uint32_t w, uint32_t h, uint8_t* out, hipStream_t s);
+// The elementary-operation check (tx_util_elementary): d_tables holds
+// 256 + 17 + 4096 floats; the three per-value outputs hold n entries each.
+hipError_t launch_elementary(const float* d_in, uint32_t n, float* d_tables, float* d_snap31, float* d_snap63,
+                             int32_t* d_round255, hipStream_t s);
 // One mip step of the RGBA8 image src (w x h, 4-byte aligned) into dst
 // (max(1, w/2) x max(1, h/2)): 2x2 box, edge clamp, (sum + 2) >> 2.
 hipError_t launch_mip_down(const uint8_t* src, uint32_t w, uint32_t h, uint8_t* dst, hipStream_t s);

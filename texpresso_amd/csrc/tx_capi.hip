@@ -614,6 +614,45 @@ int tx_util_copy(void* d_dst, const void* d_src, size_t bytes)
     return hipStreamSynchronize(ctx->stream) == hipSuccess ? TX_OK : TX_ERR_HIP;
 }
 
+int tx_util_elementary(const float* in, size_t n, float u8f[256], float sqrt_counts[17], float sqrt_alpha[4096],
+                       float* snap31, float* snap63, int32_t* round255)
+{
+    if (!u8f || !sqrt_counts || !sqrt_alpha || n > ((size_t)1 << 26) || (n && (!in || !snap31 || !snap63 || !round255)))
+        return TX_ERR_INVALID;
+    int rc = TX_OK;
+    txc::DeviceContext* ctx = txc::current_context(&rc);
+    if (!ctx) return rc;
+    constexpr size_t kTables = 256 + 17 + 4096;
+    const size_t m = n ? n : 1;
+    uint8_t* d = nullptr;
+    const size_t bytes = 4 * (kTables + 4 * m);
+    if (hipMalloc(&d, bytes) != hipSuccess) return TX_ERR_NOMEM;
+    float* d_tables = reinterpret_cast<float*>(d);
+    float* d_in = d_tables + kTables;
+    float* d_s31 = d_in + m;
+    float* d_s63 = d_s31 + m;
+    int32_t* d_r = reinterpret_cast<int32_t*>(d_s63 + m);
+    rc = TX_OK;
+    if (n && hipMemcpyAsync(d_in, in, 4 * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = TX_ERR_HIP;
+    if (rc == TX_OK && txk::launch_elementary(d_in, (uint32_t)n, d_tables, d_s31, d_s63, d_r, ctx->stream) != hipSuccess)
+        rc = TX_ERR_HIP;
+    if (rc == TX_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = TX_ERR_HIP;
+    float tables[kTables];
+    if (rc == TX_OK && hipMemcpy(tables, d_tables, sizeof tables, hipMemcpyDeviceToHost) != hipSuccess) rc = TX_ERR_HIP;
+    if (rc == TX_OK && n &&
+        (hipMemcpy(snap31, d_s31, 4 * n, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(snap63, d_s63, 4 * n, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(round255, d_r, 4 * n, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = TX_ERR_HIP;
+    if (rc == TX_OK) {
+        memcpy(u8f, tables, 4 * 256);
+        memcpy(sqrt_counts, tables + 256, 4 * 17);
+        memcpy(sqrt_alpha, tables + 256 + 17, 4 * 4096);
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
 int tx_util_time_compress(int format, const uint8_t* d_rgba, size_t width, size_t height, const tx_params* params,
                           uint8_t* d_output, size_t out_len, int iters, float* ms_total)
 {
