@@ -8,8 +8,9 @@ BC3 with Algorithm::IterativeClusterFit on a seeded 2048x2048 RGBA8 texture
 resident in HBM.  `value` = unpadded pixels / mean kernel time per step.
 
 Also measured in the same run: BC5 on a 2048x2048 seeded normal map, a
-byte-for-byte parity check of the full BC3 output against the CPU oracle,
-and the oracle itself on the host cores as the CPU baseline.
+byte-for-byte parity check of the full BC3 output against the CPU oracle
+(blocks that differ are counted, with their RGB/alpha MSE delta against the
+source), and the oracle itself on the host cores as the CPU baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -151,6 +152,39 @@ def op_rates():
     return rows
 
 
+def parity_report(O, fmt, img, w, h, got, want):
+    """Blocks whose bytes differ from the oracle's, and over those blocks the
+    RGB and alpha mean squared error against the source pixels of the GPU
+    block's decode minus that of the oracle block's decode (decoded with the
+    oracle's decoder; 0 when no block differs)."""
+    bs = 8 if fmt in (0, 3) else 16
+    g = np.asarray(got, np.uint8).reshape(-1, bs)
+    r = np.asarray(want, np.uint8).reshape(-1, bs)
+    diff = np.flatnonzero(np.any(g != r, axis=1))
+    rep = {"blocks": int(r.shape[0]), "mismatched_blocks": int(diff.size), "rgb_mse_delta": 0.0,
+           "alpha_mse_delta": 0.0}
+    if diff.size:
+        src = np.asarray(img, np.uint8).reshape(h, w, 4).astype(np.float64)
+        bw = (w + 3) // 4
+        d_rgb, d_a = [], []
+        for b in diff:
+            bx, by = int(b) % bw, int(b) // bw
+            px = [(4 * by + y, 4 * bx + x, 4 * y + x) for y in range(4) for x in range(4)
+                  if 4 * by + y < h and 4 * bx + x < w]
+            if not px:
+                continue
+            dg = O.decompress_block(fmt, g[b]).reshape(16, 4).astype(np.float64)
+            dr = O.decompress_block(fmt, r[b]).reshape(16, 4).astype(np.float64)
+            s = np.array([src[yy, xx] for yy, xx, _ in px])
+            ig = dg[[i for _, _, i in px]]
+            ir = dr[[i for _, _, i in px]]
+            d_rgb.append(np.mean((ig[:, :3] - s[:, :3]) ** 2) - np.mean((ir[:, :3] - s[:, :3]) ** 2))
+            d_a.append(np.mean((ig[:, 3] - s[:, 3]) ** 2) - np.mean((ir[:, 3] - s[:, 3]) ** 2))
+        rep["rgb_mse_delta"] = round(float(np.mean(d_rgb)), 6) if d_rgb else 0.0
+        rep["alpha_mse_delta"] = round(float(np.mean(d_a)), 6) if d_a else 0.0
+    return rep
+
+
 def time_device(tx, fmt, d_in, w, h, params, d_out, out_len, steps, warmup):
     """Mean kernel milliseconds per step from HIP events recorded on the
     library stream around `steps` launches, plus the wall time around them."""
@@ -208,14 +242,15 @@ def bc1_cluster_cpu(O, g, threads):
     t0 = time.perf_counter()
     want, stats = O.compress(0, g["img"], W, H, op, threads=threads, stats=True)
     t_cpu = time.perf_counter() - t0
-    bad = int(np.count_nonzero(np.any(g["got"].reshape(-1, 8) != want.reshape(-1, 8), axis=1)))
+    par = parity_report(O, 0, g["img"], W, H, g["got"], want)
     flops = algorithmic_flops(stats)
     ms = g["ms"]
     gpu = W * H / 1e6 / (ms / 1e3)
     cpu = W * H / 1e6 / t_cpu
     return {"config": "bc1-%d: BC1 ClusterFit %dx%d" % (W, W, H), "gpu_mpix_s": round(gpu, 1),
             "ms": round(ms, 4), "cpu_mpix_s": round(cpu, 3), "cpu_threads": threads,
-            "gpu_over_cpu": round(gpu / cpu, 1), "mismatched_blocks": bad, "blocks": W * H // 16,
+            "gpu_over_cpu": round(gpu / cpu, 1), "mismatched_blocks": par["mismatched_blocks"],
+            "rgb_mse_delta": par["rgb_mse_delta"], "alpha_mse_delta": par["alpha_mse_delta"], "blocks": W * H // 16,
             "input_sha256": hashlib.sha256(g["img"].tobytes()).hexdigest()[:16],
             "valu_tflops": round(flops / (ms / 1e3) / 1e12, 2)}
 
@@ -288,6 +323,15 @@ def chain_cpu(O, chain, g, threads):
         return sum(int(np.count_nonzero(np.any(o.reshape(-1, bs) != w.reshape(-1, bs), axis=1)))
                    for o, w in zip(outs, want))
 
+    def mse_deltas(outs):
+        reps = [parity_report(O, g["fmt"], l, l.shape[1], l.shape[0], o, w) for l, o, w in zip(chain, outs, want)
+                if np.any(o != w)]
+        n = sum(r["mismatched_blocks"] for r in reps)
+        if not n:
+            return 0.0, 0.0
+        return (round(sum(r["rgb_mse_delta"] * r["mismatched_blocks"] for r in reps) / n, 6),
+                round(sum(r["alpha_mse_delta"] * r["mismatched_blocks"] for r in reps) / n, 6))
+
     traffic = 4 * px + sum(len(w) for w in want)
     ms_batch, ms_plain = g["ms_batch"], g["ms_plain"]
     return {"config": "chains: %s %s 4096 chain (%d levels)" % (g["name"], g["alg"].name, len(chain)),
@@ -301,6 +345,7 @@ def chain_cpu(O, chain, g, threads):
             "gpu_over_cpu": round((px / ms_batch / 1e3) / (px / t_cpu / 1e6), 1),
             "mismatched_blocks": bad_blocks(g["got"]), "mismatched_blocks_per_level": bad_blocks(g["got_plain"]),
             "mismatched_blocks_per_level_onepass": bad_blocks(g["got_onepass"]),
+            "rgb_alpha_mse_delta_batched": mse_deltas(g["got"]),
             "tuning": g["tuning"],
             "valu_tflops": round(flops / (ms_batch / 1e3) / 1e12, 3),
             "valu_frac_of_157": round(flops / (ms_batch / 1e3) / 1e12 / VALU_PEAK_TFLOPS, 4),
@@ -714,8 +759,7 @@ def main():
         threads, online, affinity = cpu_counts()
         op = O.params(int(params.algorithm), params.weights, False)
         want, stats = O.compress(int(fmt), img, W, H, op, threads=threads, stats=True)
-        bad = int(np.count_nonzero(np.any(gpu_out.reshape(-1, 16) != want.reshape(-1, 16), axis=1)))
-        line["parity"] = {"blocks": int(want.size // 16), "mismatched_blocks": bad}
+        line["parity"] = parity_report(O, int(fmt), img, W, H, gpu_out, want)
         flops = algorithmic_flops(stats)
         workload = line["config"]["workload"]
         if phases is not None:
