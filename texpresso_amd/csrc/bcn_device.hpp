@@ -543,13 +543,19 @@ __device__ __forceinline__ float eval_split(const SplitSums& s, const float4& xs
             ab = (2.0f / 9.0f) * (p1.w + p2.w);
         }
     } else {
-        // part2 = xsum - part1 - part0; h = part1*(.5,.5,.5,.25)
+        // part2 = xsum - part1 - part0; alphax = part1*(.5,.5,.5) + part0,
+        // betax = part1*(.5,.5,.5) + part2.  Halving is exact for the normal
+        // prefix sums, so the fused form rounds once exactly as the
+        // product-then-add does.
         const float p2x = (xs.x - p1.x) - p0.x;
         const float p2y = (xs.y - p1.y) - p0.y;
         const float p2z = (xs.z - p1.z) - p0.z;
-        const float hx = p1.x * 0.5f, hy = p1.y * 0.5f, hz = p1.z * 0.5f;
-        ax[0] = hx + p0.x; ax[1] = hy + p0.y; ax[2] = hz + p0.z;
-        bx[0] = hx + p2x; bx[1] = hy + p2y; bx[2] = hz + p2z;
+        ax[0] = __builtin_fmaf(0.5f, p1.x, p0.x);
+        ax[1] = __builtin_fmaf(0.5f, p1.y, p0.y);
+        ax[2] = __builtin_fmaf(0.5f, p1.z, p0.z);
+        bx[0] = __builtin_fmaf(0.5f, p1.x, p2x);
+        bx[1] = __builtin_fmaf(0.5f, p1.y, p2y);
+        bx[2] = __builtin_fmaf(0.5f, p1.z, p2z);
         if (UNIT) {
             a2 = u.a2;
             b2 = u.b2;
