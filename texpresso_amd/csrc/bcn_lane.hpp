@@ -104,11 +104,83 @@ __device__ __forceinline__ uint32_t match_mask16(uint32_t key, const uint32_t (&
     return mask;
 }
 
+// Batcher's odd-even merge sort of 16 values, ascending: 63 compare-exchange
+// steps, straight-line once unrolled (the network sorts every 0/1 input, so
+// it sorts every input).
+__device__ __forceinline__ void sort16(uint32_t (&k)[16])
+{
+#pragma unroll
+    for (int p = 1; p < 16; p <<= 1)
+#pragma unroll
+        for (int d = p; d >= 1; d >>= 1)
+#pragma unroll
+            for (int j = d % p; j + d < 16; j += 2 * d)
+#pragma unroll
+                for (int i = 0; i < d; ++i)
+                    if (i + j + d < 16 && (i + j) / (2 * p) == (i + j + d) / (2 * p)) {
+                        const uint32_t a = k[i + j], b = k[i + j + d];
+                        k[i + j] = min(a, b);
+                        k[i + j + d] = max(a, b);
+                    }
+}
+
+// colourset.rs:35-112 on one lane, unweighted points (weigh_colour_by_alpha
+// off): the 16 keys (colour << 4 | pixel; an excluded pixel's colour is a
+// value no colour has) sorted put each colour's pixels in one run, lowest
+// pixel first.  The run's first pixel opens the colour's point when it is in
+// the set (it is the lowest pixel of that colour); every pixel of the run has
+// it as its lowest same-colour pixel; the point's weight is sqrtf(run
+// length).  Same sets, remaps and weights as the pairwise compares of the
+// weighted form below.
+template <int FMT>
+__device__ __forceinline__ void colour_set_lane_sorted(const uint32_t (&px)[16], uint32_t inc, const LaneTables& T,
+                                                       LaneBlockLds& L, int lane, LaneSet& S)
+{
+    uint32_t k[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) k[p] = (colour_key(px[p], inc, p) << 4) | (uint32_t)p;
+    sort16(k);
+    uint32_t first = 0, head = 0, cnt = 0;
+    uint64_t lowest = 0;  // nibble p: the lowest pixel of pixel p's colour
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const uint32_t i = k[s] & 15u;
+        const bool start = s == 0 || (k[s] >> 4) != (k[s - 1] >> 4);
+        head = start ? i : head;
+        cnt = start ? 1u : cnt + 1u;
+        const bool in = (inc >> i) & 1u;
+        first |= (start && in ? 1u : 0u) << i;
+        lowest |= (uint64_t)head << (4 * i);
+        // weights: the run's first pixel holds sqrt(pixels so far), so its
+        // run's last pixel leaves the full count; every other pixel 0, and an
+        // excluded pixel (a run of its own) 0
+        L.w[i][lane] = start && in ? T.sqc[1] : 0.0f;
+        if (!start) L.w[head][lane] = T.sqc[cnt];
+    }
+    uint64_t remap = 0;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const uint32_t f = (uint32_t)(lowest >> (4 * p)) & 15u;
+        const uint64_t r = ((inc >> p) & 1u) ? (uint64_t)__popc(first & ((1u << f) - 1u)) : 255u;
+        remap |= r << (8 * (p & 7));
+        if (p == 7) {
+            S.remap[0] = (uint32_t)remap;
+            S.remap[1] = (uint32_t)(remap >> 32);
+            remap = 0;
+        }
+    }
+    S.remap[2] = (uint32_t)remap;
+    S.remap[3] = (uint32_t)(remap >> 32);
+    S.first = first;
+    S.n = __popc(first);
+}
+
 // colourset.rs:35-112 on one lane.  For each pixel p, same = the pixels with
 // p's colour (p included): p opens a point when none of them is lower; its
 // point index counts the points opened below its lowest same-colour pixel;
 // the point's weight is the pixel count, or the sum of (alpha + 1) / 256
-// over those pixels (exact in binary32), then sqrtf.
+// over those pixels (exact in binary32), then sqrtf.  Unweighted points take
+// the sorted form above.
 template <int FMT>
 __device__ __forceinline__ void colour_set_lane(const uint32_t (&px)[16], uint32_t valid, const FitParams& fp,
                                                 const LaneTables& T, LaneBlockLds& L, int lane, LaneSet& S)
@@ -126,6 +198,10 @@ __device__ __forceinline__ void colour_set_lane(const uint32_t (&px)[16], uint32
     }
     S.inc = inc;
     S.transparent = FMT == BC1 && transp != 0;
+    if (!fp.weigh_alpha) {
+        colour_set_lane_sorted<FMT>(px, inc, T, L, lane, S);
+        return;
+    }
     uint32_t first = 0;
     uint64_t remap = 0;
     uint32_t keys[16];
