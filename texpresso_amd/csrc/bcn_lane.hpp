@@ -249,9 +249,11 @@ __device__ __forceinline__ void colour_set_lane(const uint32_t (&px)[16], uint32
 // math.rs:44-97 over the points of a lane's set (same folds as
 // principal_axis).  The folds run over all 16 pixels: a pixel that opens no
 // point has w = 0, so its terms are +0 or -0 and adding them leaves the
-// (never negative-zero) accumulators unchanged.
+// (never negative-zero) accumulators unchanged.  With `pts`, the first fold
+// also stores each point as (x, y, z, weight) at its point index there (the
+// ClusterFit record of the block).
 __device__ __forceinline__ void principal_axis_lane(const LaneSet& S, const LaneTables& T, float& vx, float& vy,
-                                                    float& vz)
+                                                    float& vz, float4* __restrict__ pts = nullptr)
 {
     float total = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 #pragma unroll 4
@@ -259,6 +261,7 @@ __device__ __forceinline__ void principal_axis_lane(const LaneSet& S, const Lane
         float x, y, z;
         xyz(T, S.pix(p), x, y, z);
         const float w = S.wt(p);
+        if (pts && ((S.first >> p) & 1u)) pts[__popc(S.first & ((1u << p) - 1u))] = make_float4(x, y, z, w);
         total = total + w;
         cx = cx + x * w;
         cy = cy + y * w;
@@ -485,7 +488,8 @@ __device__ __forceinline__ void bc2_alpha_lane(const uint32_t (&px)[16], uint32_
 template <int FMT, int ALG>
 __device__ __forceinline__ int encode_front_lane(const uint32_t (&px)[16], uint32_t valid, const FitParams& fp,
                                                  const LaneTables& T, LaneBlockLds& L, int lane, uint32_t (&out)[4],
-                                                 LaneSet& S, float& vx, float& vy, float& vz)
+                                                 LaneSet& S, float& vx, float& vy, float& vz,
+                                                 float4* __restrict__ cluster_pts = nullptr)
 {
     out[0] = out[1] = out[2] = out[3] = 0;
     colour_set_lane<FMT>(px, valid, fp, T, L, lane, S);
@@ -501,7 +505,7 @@ __device__ __forceinline__ int encode_front_lane(const uint32_t (&px)[16], uint3
     uint32_t lo = 0, hi = 0;
     vx = vy = vz = 0.0f;
     if (kind == KIND_SINGLE) single_fit_lane<FMT>(S, T, lo, hi);
-    if (kind != KIND_SINGLE) principal_axis_lane(S, T, vx, vy, vz);
+    if (kind != KIND_SINGLE) principal_axis_lane(S, T, vx, vy, vz, kind == KIND_CLUSTER ? cluster_pts : nullptr);
     if (kind == KIND_RANGE) range_fit_lane<FMT>(S, T, fp, vx, vy, vz, lo, hi);
     set_colour<FMT>(out, lo, hi);
     return kind;

@@ -85,22 +85,15 @@ __device__ __forceinline__ void count_keys(PipeCtl* __restrict__ ctl, uint32_t k
     }
 }
 
-// ClusterFit block on one lane: its record into slot r, and its key.
+// ClusterFit block on one lane: the rest of its record in slot r (the points
+// were written by the front's centroid fold), and its key.
 template <int FMT, int ALG>
 __device__ __forceinline__ uint32_t write_record_lane(ClusterRec* __restrict__ recs, uint32_t r,
-                                                      const LaneSet& S, const LaneTables& T, const uint32_t (&out)[4],
+                                                      const LaneSet& S, const uint32_t (&out)[4],
                                                       float vx, float vy, float vz, const FitParams& fp,
                                                       uint32_t tex_idx, uint32_t block)
 {
-    ClusterRec& R = recs[r];
-#pragma unroll 2
-    for (int p = 0; p < 16; ++p) {
-        if ((S.first >> p) & 1u) {
-            float x, y, z;
-            xyz(T, S.pix(p), x, y, z);
-            R.pts[__popc(S.first & ((1u << p) - 1u))] = make_float4(x, y, z, S.wt(p));
-        }
-    }
+    ClusterRec& R = recs[r];  // points: written by the front's centroid fold
     const bool need4 = !(FMT == BC1 && S.transparent);
     const uint32_t info = (uint32_t)S.n | ((S.transparent ? 1u : 0u) << 8) | ((uint32_t)FMT << 12) |
                           ((ALG == ITERATIVE_CLUSTER_FIT ? 8u : 1u) << 16);
@@ -143,9 +136,10 @@ __device__ __forceinline__ void classify_chunk(const BatchTex& T, uint32_t tex_i
             uint32_t w[4];
             LaneSet S;
             float vx, vy, vz;
-            const int kind = encode_front_lane<FMT, ALG>(px, valid, fp, LT, LB, lane, w, S, vx, vy, vz);
+            const int kind = encode_front_lane<FMT, ALG>(px, valid, fp, LT, LB, lane, w, S, vx, vy, vz,
+                                                         recs[slot0 + k].pts);
             if (kind == KIND_CLUSTER) {
-                key = write_record_lane<FMT, ALG>(recs, slot0 + k, S, LT, w, vx, vy, vz, fp, tex_idx, b);
+                key = write_record_lane<FMT, ALG>(recs, slot0 + k, S, w, vx, vy, vz, fp, tex_idx, b);
             } else if (FMT == BC1) {
                 *reinterpret_cast<uint2*>(T.out + (size_t)b * 8u) = make_uint2(w[0], w[1]);
             } else {
