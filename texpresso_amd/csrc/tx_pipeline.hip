@@ -39,6 +39,7 @@ using txp::kLists;
 using txp::kListA;
 using txp::kListB;
 using txp::kListC;
+using txp::kSearchWaves;
 constexpr uint32_t key_bit(int list) { return 1u << (5 + list); }
 
 __global__ void k_pipe_reset(PipeCtl* ctl)
@@ -286,6 +287,52 @@ __device__ __forceinline__ void emit_block(const BatchTex* __restrict__ tex, con
     }
 }
 
+// Takes `nwant` (1..4) list positions for one wave from its workgroup's run
+// [next, end), packed as (end << 32) | next in LDS and shared by the
+// workgroup's 16 groups.  Executed by one lane.  A wave that finds the run
+// empty takes the refill lock, reserves the next `refill` positions from
+// the shared cursor and installs them; the other waves wait for the lock
+// to drop (its holder waits on nothing).  The positions come in up to
+// nwant consecutive pieces and are returned in order in pos[0..nwant).
+// Positions past the list's end are handed out like any other; their
+// groups go idle.
+__device__ __forceinline__ void take_positions(uint64_t* run, uint32_t* refilling, uint32_t* __restrict__ cursor,
+                                               uint32_t cursor_base, uint32_t refill, uint32_t nwant,
+                                               uint32_t (&pos)[4])
+{
+    uint32_t got = 0;
+    while (got < nwant) {
+        uint64_t w = __hip_atomic_load(run, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t next = (uint32_t)w, end = (uint32_t)(w >> 32);
+        if (next < end) {
+            const uint32_t k = min(nwant - got, end - next);
+            if (__hip_atomic_compare_exchange_strong(run, &w, ((uint64_t)end << 32) | (next + k), __ATOMIC_ACQ_REL,
+                                                     __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i)
+                    if (i >= got && i < got + k) pos[i] = next + (i - got);
+                got += k;
+            }
+            continue;
+        }
+        uint32_t unlocked = 0u;
+        if (__hip_atomic_compare_exchange_strong(refilling, &unlocked, 1u, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            // only the lock holder installs a run, and no taker changes an
+            // empty one, so a run still empty here stays empty until the store
+            w = __hip_atomic_load(run, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if ((uint32_t)w >= (uint32_t)(w >> 32)) {
+                const uint32_t fresh = atomicAdd(cursor, refill) + cursor_base;
+                __hip_atomic_store(run, ((uint64_t)(fresh + refill) << 32) | fresh, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            __hip_atomic_store(refilling, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
 // Persistent search: each 16-lane group pulls records and runs the block's
 // ordering iterations independently (cluster.rs:174-249 / 301-389).  NC = 3:
 // BC1's compress3, whose result goes to the record (or, for a BC1 block with
@@ -299,16 +346,38 @@ __device__ __forceinline__ void emit_block(const BatchTex* __restrict__ tex, con
 // remain.  Search phase: every group with a fresh ordering evaluates its
 // candidate splits; the candidate loop runs as long as the largest colour
 // count among them needs, and the records are sorted by colour count.
+//
+// A workgroup is kSearchWaves waves (4 groups each) sharing one run of list
+// positions: a group takes one position at a time from it, so near the end
+// of the list no group holds records that an idle group could have taken,
+// while the shared cursor still sees one atomic per 16 positions.  The waves
+// are otherwise independent (no workgroup barrier after the set-up).
 template <int NC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_search(const BatchTex* __restrict__ tex, BatchTex tex0,
-                                                    ClusterRec* __restrict__ recs, const uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor)
+__global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_search(
+    const BatchTex* __restrict__ tex, BatchTex tex0, ClusterRec* __restrict__ recs, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor)
 {
-    __shared__ GroupLds lds[4];
-    const int lane = (int)threadIdx.x, g = lane >> 4, l = lane & 15;
-    GroupLds& L = lds[g];
+    __shared__ GroupLds lds[4 * kSearchWaves];
+    __shared__ uint64_t run;
+    __shared__ uint32_t refilling;
+    const int lane = (int)(threadIdx.x & 63u), g = lane >> 4, l = lane & 15;
+    GroupLds& L = lds[4 * (threadIdx.x >> 6) + g];
     uint32_t total = 0;
     for (int m = 0; m < 17; ++m) total += hist[m];
+    // refills of kRefill positions: at most one per group of the workgroup
+    // and at most the records per wave, so a short list still spreads over
+    // every workgroup; the
+    // first run of each workgroup is fixed by its index (no atomic burst at
+    // launch) and refills come from the shared cursor, offset past those runs
+    const uint32_t kRefill = min(16u, max(4u, total / (kSearchWaves * gridDim.x)));
+    const uint32_t kFirst = kSearchWaves * kRefill;
+    const uint32_t cursor_base = gridDim.x * kFirst;
+    if (threadIdx.x == 0) {
+        const uint32_t first = blockIdx.x * kFirst;
+        run = ((uint64_t)(first + kFirst) << 32) | first;
+        refilling = 0u;
+    }
+    __syncthreads();
 
     bool idle = false, need = true, ready = false;
     uint32_t r = 0, info = 0, lb_code = 0;
@@ -317,14 +386,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     float sa[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};
     float axx = 0.0f, axy = 0.0f, axz = 0.0f;
     FitParams fp{0.0f, 0.0f, 0.0f, 0u};
-    // the wave's private run of list positions [pool, pool_end), refilled
-    // kPull at a time with one global atomic; kPull is at most the records
-    // per wave, so a short list still spreads over every wave
-    const uint32_t kPull = min(16u, max(4u, total / gridDim.x));
-    // the first run is fixed by the workgroup index (no atomic burst at
-    // launch); refills come from the shared cursor, offset past those runs
-    uint32_t pool = blockIdx.x * kPull, pool_end = pool + kPull;
-    const uint32_t cursor_base = gridDim.x * kPull;
     for (;;) {
         // ---- ordering phase
         for (;;) {
@@ -334,20 +395,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const uint64_t want = __ballot(take && l == 0);
                 const uint32_t nwant = (uint32_t)__popcll(want);
                 const uint32_t rank = (uint32_t)__popcll(want & ((1ull << (uint32_t)(g * 16)) - 1ull));
-                const uint32_t avail = pool_end - pool;
-                uint32_t p = pool + rank;
-                if (avail < nwant) {
-                    // reserve enough for every asking group (nwant <= 4)
-                    const uint32_t grab = max(kPull, nwant);
-                    uint32_t fresh = 0;
-                    if (lane == 0) fresh = atomicAdd(cursor, grab);
-                    fresh = __builtin_amdgcn_readfirstlane(fresh) + cursor_base;
-                    if (rank >= avail) p = fresh + (rank - avail);
-                    pool = fresh + (nwant - avail);
-                    pool_end = fresh + grab;
-                } else {
-                    pool += nwant;
-                }
+                uint32_t pos[4] = {0u, 0u, 0u, 0u};
+                if (lane == 0) take_positions(&run, &refilling, cursor, cursor_base, kRefill, nwant, pos);
+                const uint32_t p0 = __builtin_amdgcn_readfirstlane(pos[0]), p1 = __builtin_amdgcn_readfirstlane(pos[1]);
+                const uint32_t p2 = __builtin_amdgcn_readfirstlane(pos[2]), p3 = __builtin_amdgcn_readfirstlane(pos[3]);
+                const uint32_t p = rank == 0u ? p0 : rank == 1u ? p1 : rank == 2u ? p2 : p3;
                 if (take) {
                     if (p >= total) {
                         idle = true;
@@ -510,6 +562,7 @@ hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s
                        p.nchunks, p.chunk_blocks, b.recs, b.keys, b.ctl);
     if (mark(1) != hipSuccess) return hipErrorUnknown;
     const uint32_t bin_grid = (uint32_t)((p.nslots + kBinSeg - 1) / kBinSeg);
+    const uint32_t search_wgs = (p.search_grid + kSearchWaves - 1) / kSearchWaves;
     hipLaunchKernelGGL(txd::k_pipe_bin, dim3(bin_grid), dim3(256), 0, s, b.keys, p.nslots, b.ctl, b.list[kListA],
                        b.list[kListB], b.list[kListC]);
     if (mark(2) != hipSuccess) return hipErrorUnknown;
@@ -517,11 +570,11 @@ hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s
         hipLaunchKernelGGL(txd::k_pipe_cluster, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
                            b.list[kListC], b.ctl->hist[kListC], &b.ctl->cursor[kListC]);
     if (p.lists & (1u << kListA))
-        hipLaunchKernelGGL(txd::k_pipe_search<3>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
+        hipLaunchKernelGGL(txd::k_pipe_search<3>, dim3(search_wgs), dim3(64 * kSearchWaves), 0, s, p.d_tex, p.tex0, b.recs,
                            b.list[kListA], b.ctl->hist[kListA], &b.ctl->cursor[kListA]);
     if (mark(3) != hipSuccess) return hipErrorUnknown;
     if (p.lists & (1u << kListB))
-        hipLaunchKernelGGL(txd::k_pipe_search<4>, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
+        hipLaunchKernelGGL(txd::k_pipe_search<4>, dim3(search_wgs), dim3(64 * kSearchWaves), 0, s, p.d_tex, p.tex0, b.recs,
                            b.list[kListB], b.ctl->hist[kListB], &b.ctl->cursor[kListB]);
     if (mark(4) != hipSuccess) return hipErrorUnknown;
     return hipGetLastError();

@@ -91,12 +91,14 @@ struct PipeBufs {
     void carve(uint8_t* base, uint64_t cap);
 };
 
-// Resident single-wave workgroups per CU: classify (one block per lane) is
-// limited by registers and LDS (8 KiB of pixel and weight copies per
-// workgroup), the searches by LDS and registers (4 x 2512 B per workgroup);
-// both fit 4 waves per SIMD.
+// Resident waves per CU: classify (single-wave workgroups, one block per
+// lane) is limited by registers and LDS (8 KiB of pixel and weight copies
+// per workgroup), the searches by LDS and registers (4 x 2512 B per wave);
+// both fit 4 waves per SIMD.  The 16-colour searches run kSearchWaves waves
+// per workgroup (k_pipe_search), ClusterFit's single wave (k_pipe_cluster).
 constexpr uint32_t kClassifyWavesPerCu = 16;
 constexpr uint32_t kSearchWavesPerCu = 16;
+constexpr uint32_t kSearchWaves = 4;  // waves per search workgroup (sharing one run of list positions)
 
 struct PipeLaunch {
     const txk::BatchTex* d_tex;  // texture descriptors (null: only `tex0`)
