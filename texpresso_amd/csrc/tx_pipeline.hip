@@ -479,31 +479,39 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
 // BC1; compress4 for BC2/BC3.  With one ordering per search there is nothing
 // to gain from letting the groups run independently, and for BC1 one pull
 // serves both searches.  Records are sorted by colour count, so the four
-// candidate loops have the same length.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_cluster(
+// candidate loops have the same length.  As in k_pipe_search, the
+// workgroup's kSearchWaves waves take their positions, four per step, from
+// one shared run, so at the end of the list no wave holds steps that an
+// idle wave could have run.
+__global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_pipe_cluster(
     const BatchTex* __restrict__ tex, BatchTex tex0, const ClusterRec* __restrict__ recs,
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor)
 {
-    __shared__ GroupLds lds[4];
-    const int lane = (int)threadIdx.x, g = lane >> 4, l = lane & 15;
-    GroupLds& L = lds[g];
+    __shared__ GroupLds lds[4 * kSearchWaves];
+    __shared__ uint64_t run;
+    __shared__ uint32_t refilling;
+    const int lane = (int)(threadIdx.x & 63u), g = lane >> 4, l = lane & 15;
+    GroupLds& L = lds[4 * (threadIdx.x >> 6) + g];
     uint32_t total = 0;
     for (int m = 0; m < 17; ++m) total += hist[m];
-    // the wave takes four list positions per step (one per group) from a
-    // private run of kPull; the first run is fixed by the workgroup index,
-    // refills come from the shared cursor, offset past those runs
-    const uint32_t kPull = 4u * min(4u, max(1u, total / (4u * gridDim.x)));
-    uint32_t pool = blockIdx.x * kPull, pool_end = pool + kPull;
-    const uint32_t cursor_base = gridDim.x * kPull;
+    // refills of up to four steps for one wave (16 positions), at most the
+    // records per wave; the first run of each workgroup is fixed by its
+    // index, refills come from the shared cursor, offset past those runs
+    const uint32_t kRefill = 4u * min(4u, max(1u, total / (4u * kSearchWaves * gridDim.x)));
+    const uint32_t kFirst = kSearchWaves * kRefill;
+    const uint32_t cursor_base = gridDim.x * kFirst;
+    if (threadIdx.x == 0) {
+        const uint32_t first = blockIdx.x * kFirst;
+        run = ((uint64_t)(first + kFirst) << 32) | first;
+        refilling = 0u;
+    }
+    __syncthreads();
     for (;;) {
-        if (pool == pool_end) {
-            uint32_t fresh = 0;
-            if (lane == 0) fresh = atomicAdd(cursor, kPull);
-            pool = __builtin_amdgcn_readfirstlane(fresh) + cursor_base;
-            pool_end = pool + kPull;
-        }
-        const uint32_t p = pool + (uint32_t)g;
-        pool += 4;
+        uint32_t pos[4] = {0u, 0u, 0u, 0u};
+        if (lane == 0) take_positions(&run, &refilling, cursor, cursor_base, kRefill, 4u, pos);
+        const uint32_t p0 = __builtin_amdgcn_readfirstlane(pos[0]), p1 = __builtin_amdgcn_readfirstlane(pos[1]);
+        const uint32_t p2 = __builtin_amdgcn_readfirstlane(pos[2]), p3 = __builtin_amdgcn_readfirstlane(pos[3]);
+        const uint32_t p = g == 0 ? p0 : g == 1 ? p1 : g == 2 ? p2 : p3;
         const bool has = p < total;
         if (!wave_any(has)) break;
         ColourSet cs{0, -1, false};
@@ -567,7 +575,7 @@ hipError_t launch_pipeline(const PipeLaunch& p, const PipeBufs& b, hipStream_t s
                        b.list[kListB], b.list[kListC]);
     if (mark(2) != hipSuccess) return hipErrorUnknown;
     if (p.lists & (1u << kListC))
-        hipLaunchKernelGGL(txd::k_pipe_cluster, dim3(p.search_grid), dim3(64), 0, s, p.d_tex, p.tex0, b.recs,
+        hipLaunchKernelGGL(txd::k_pipe_cluster, dim3(search_wgs), dim3(64 * kSearchWaves), 0, s, p.d_tex, p.tex0, b.recs,
                            b.list[kListC], b.ctl->hist[kListC], &b.ctl->cursor[kListC]);
     if (p.lists & (1u << kListA))
         hipLaunchKernelGGL(txd::k_pipe_search<3>, dim3(search_wgs), dim3(64 * kSearchWaves), 0, s, p.d_tex, p.tex0, b.recs,
