@@ -56,7 +56,10 @@ constexpr const char* kOpMixNames[] = {
     "8 v_add_f32",          "8 v_mul_f32",          "8 v_fma_f32",         "8 v_add_u32",
     "8 v_sad_u16",          "8 v_min3_u32",         "8 v_trunc_f32",       "8 v_bfe_u32",
     "8 v_mul_u32_u24",      "4 v_sad_u16 + 4 v_add_f32", "4 v_trunc_f32 + 4 v_mul_f32",
-    "4 v_min3_u32 + 4 v_add_u32"};
+    "4 v_min3_u32 + 4 v_add_u32", "8 v_mul_f32_e64 (VOP3)", "8 v_mul_f32 clamp",
+    "4 v_mul_f32 clamp + 4 v_add_f32", "8 v_med3_f32",     "8 v_max_f32",         "8 v_floor_f32",
+    "8 v_fract_f32",        "8 v_cvt_u32_f32",      "8 v_cndmask_b32",     "8 v_sub_f32",
+    "4 v_trunc_f32 + 4 v_add_f32", "4 v_cndmask_b32 + 4 v_mul_f32"};
 constexpr int kOpMixes = (int)(sizeof(kOpMixNames) / sizeof(kOpMixNames[0]));
 
 template <int M>
@@ -79,6 +82,18 @@ __global__ __launch_bounds__(256) void k_op_rate(uint32_t* out, int iters, uint3
         if (M == 9) { TXB_MIX2("v_sad_u16 %0, %0, %1, %2", "v_add_f32 %0, %0, %1"); }
         if (M == 10) { TXB_MIX2("v_trunc_f32 %0, %0", "v_mul_f32 %0, %0, %1"); }
         if (M == 11) { TXB_MIX2("v_min3_u32 %0, %0, %1, %2", "v_add_u32 %0, %0, %1"); }
+        if (M == 12) { TXB_MIX2("v_mul_f32_e64 %0, %0, %1", "v_mul_f32_e64 %0, %0, %1"); }
+        if (M == 13) { TXB_MIX2("v_mul_f32_e64 %0, %0, %1 clamp", "v_mul_f32_e64 %0, %0, %1 clamp"); }
+        if (M == 14) { TXB_MIX2("v_mul_f32_e64 %0, %0, %1 clamp", "v_add_f32 %0, %0, %1"); }
+        if (M == 15) { TXB_MIX2("v_med3_f32 %0, %0, %1, %2", "v_med3_f32 %0, %0, %1, %2"); }
+        if (M == 16) { TXB_MIX2("v_max_f32 %0, %0, %1", "v_max_f32 %0, %0, %1"); }
+        if (M == 17) { TXB_MIX2("v_floor_f32 %0, %0", "v_floor_f32 %0, %0"); }
+        if (M == 18) { TXB_MIX2("v_fract_f32 %0, %0", "v_fract_f32 %0, %0"); }
+        if (M == 19) { TXB_MIX2("v_cvt_u32_f32 %0, %0", "v_cvt_u32_f32 %0, %0"); }
+        if (M == 20) { TXB_MIX2("v_cndmask_b32 %0, %0, %1, vcc", "v_cndmask_b32 %0, %0, %1, vcc"); }
+        if (M == 21) { TXB_MIX2("v_sub_f32 %0, %0, %1", "v_sub_f32 %0, %0, %1"); }
+        if (M == 22) { TXB_MIX2("v_trunc_f32 %0, %0", "v_add_f32 %0, %0, %1"); }
+        if (M == 23) { TXB_MIX2("v_cndmask_b32 %0, %0, %1, vcc", "v_mul_f32 %0, %0, %1"); }
     }
     out[t] = x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7;
 }
@@ -94,7 +109,9 @@ void launch_op_rate(int blocks, uint32_t* out, int iters, hipStream_t s)
 typedef void (*OpLaunch)(int, uint32_t*, int, hipStream_t);
 constexpr OpLaunch kOpLaunch[kOpMixes] = {
     launch_op_rate<0>, launch_op_rate<1>, launch_op_rate<2>, launch_op_rate<3>, launch_op_rate<4>,  launch_op_rate<5>,
-    launch_op_rate<6>, launch_op_rate<7>, launch_op_rate<8>, launch_op_rate<9>, launch_op_rate<10>, launch_op_rate<11>};
+    launch_op_rate<6>,  launch_op_rate<7>,  launch_op_rate<8>,  launch_op_rate<9>,  launch_op_rate<10>, launch_op_rate<11>,
+    launch_op_rate<12>, launch_op_rate<13>, launch_op_rate<14>, launch_op_rate<15>, launch_op_rate<16>, launch_op_rate<17>,
+    launch_op_rate<18>, launch_op_rate<19>, launch_op_rate<20>, launch_op_rate<21>, launch_op_rate<22>, launch_op_rate<23>};
 
 }  // namespace
 
