@@ -601,7 +601,7 @@ def main():
     # launch of the split search on the default workload); time the steady state
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--size", type=int, default=2048)
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
     ap.add_argument("--op-rates", action="store_true",
                     help="issue rate of single instruction forms and two-form mixes (benchmark probe library)")
@@ -779,7 +779,9 @@ def main():
             "algorithmic_bytes_per_step": 4 * W * H + out_len,
             "hbm_gbs": round((4 * W * H + out_len) / (ms / 1e3) / 1e9, 2), "hbm_peak_gbs": HBM_PEAK_GBS,
         }
-        # CPU baseline: the oracle on the same texture over `threads` host cores
+        # CPU baseline: the oracle on the same texture over `threads` host
+        # cores, median of --cpu-runs timed runs (the parity run above is
+        # their warm-up)
         rows = H // 4
         sample_rows = rows
         times = []
