@@ -287,6 +287,9 @@ __device__ __forceinline__ void emit_block(const BatchTex* __restrict__ tex, con
     }
 }
 
+// Lane 0's value of v (the lane that took a wave's list positions).
+__device__ __forceinline__ uint32_t lane0(uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane((int)v, 0); }
+
 // Takes `nwant` (1..4) list positions for one wave from its workgroup's run
 // [next, end), packed as (end << 32) | next in LDS and shared by the
 // workgroup's 16 groups.  Executed by one lane.  A wave that finds the run
@@ -366,9 +369,9 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
     for (int m = 0; m < 17; ++m) total += hist[m];
     // refills of kRefill positions: at most one per group of the workgroup
     // and at most the records per wave, so a short list still spreads over
-    // every workgroup; the
-    // first run of each workgroup is fixed by its index (no atomic burst at
-    // launch) and refills come from the shared cursor, offset past those runs
+    // every workgroup; the first run of each workgroup is fixed by its index
+    // (no atomic burst at launch) and refills come from the shared cursor,
+    // offset past those runs
     const uint32_t kRefill = min(16u, max(4u, total / (kSearchWaves * gridDim.x)));
     const uint32_t kFirst = kSearchWaves * kRefill;
     const uint32_t cursor_base = gridDim.x * kFirst;
@@ -397,8 +400,7 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
                 const uint32_t rank = (uint32_t)__popcll(want & ((1ull << (uint32_t)(g * 16)) - 1ull));
                 uint32_t pos[4] = {0u, 0u, 0u, 0u};
                 if (lane == 0) take_positions(&run, &refilling, cursor, cursor_base, kRefill, nwant, pos);
-                const uint32_t p0 = __builtin_amdgcn_readfirstlane(pos[0]), p1 = __builtin_amdgcn_readfirstlane(pos[1]);
-                const uint32_t p2 = __builtin_amdgcn_readfirstlane(pos[2]), p3 = __builtin_amdgcn_readfirstlane(pos[3]);
+                const uint32_t p0 = lane0(pos[0]), p1 = lane0(pos[1]), p2 = lane0(pos[2]), p3 = lane0(pos[3]);
                 const uint32_t p = rank == 0u ? p0 : rank == 1u ? p1 : rank == 2u ? p2 : p3;
                 if (take) {
                     if (p >= total) {
@@ -509,8 +511,7 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
     for (;;) {
         uint32_t pos[4] = {0u, 0u, 0u, 0u};
         if (lane == 0) take_positions(&run, &refilling, cursor, cursor_base, kRefill, 4u, pos);
-        const uint32_t p0 = __builtin_amdgcn_readfirstlane(pos[0]), p1 = __builtin_amdgcn_readfirstlane(pos[1]);
-        const uint32_t p2 = __builtin_amdgcn_readfirstlane(pos[2]), p3 = __builtin_amdgcn_readfirstlane(pos[3]);
+        const uint32_t p0 = lane0(pos[0]), p1 = lane0(pos[1]), p2 = lane0(pos[2]), p3 = lane0(pos[3]);
         const uint32_t p = g == 0 ? p0 : g == 1 ? p1 : g == 2 ? p2 : p3;
         const bool has = p < total;
         if (!wave_any(has)) break;
