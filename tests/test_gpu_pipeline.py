@@ -143,3 +143,44 @@ def test_pipeline_random_weights(pipeline_always, oracle, fmt, alg, alpha):
         want = oracle.compress(fmt, img, w, h, oracle.params(int(alg), p.weights, alpha),
                                threads=oracle.default_threads())
         assert np.array_equal(got, want), weights
+
+
+def test_pipeline_scratch_shared_across_streams(pipeline_always, oracle):
+    # The pipeline's record buffers are shared by every stream of the device:
+    # calls on callers' streams wait for the previous use and record theirs,
+    # calls on the library stream are ordered by that stream and recorded only
+    # when another stream comes next.  Alternate between two caller streams
+    # and the library stream with nothing synchronised in between, and check
+    # every texture against the oracle.
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    streams = []
+    for _ in range(2):
+        s = ctypes.c_void_p()
+        assert hip.hipStreamCreateWithFlags(ctypes.byref(s), 1) == 0  # non-blocking, default priority
+        streams.append(s)
+    try:
+        order = [streams[0], None, streams[1], None, None, streams[0], streams[1], None]
+        jobs = []
+        for k, s in enumerate(order):
+            w, h = 128, 96 + 4 * k
+            fmt = tx.Format.Bc3 if k % 2 == 0 else tx.Format.Bc1
+            img = texgen.with_alpha_field(texgen.rgb_texture(w, h, 900 + k), 40 + k)
+            p = tx.Params(tx.Algorithm.IterativeClusterFit)
+            d_in = tx.DeviceBuffer.from_array(np.ascontiguousarray(img).reshape(-1))
+            n = fmt.compressed_size(w, h)
+            d_out = tx.DeviceBuffer(n)
+            jobs.append((fmt, img, w, h, p, d_in, d_out, n))
+        for (fmt, img, w, h, p, d_in, d_out, n), s in zip(jobs, order):
+            fmt.compress_device(d_in.ptr, w, h, p, d_out.ptr, n, s)
+        assert hip.hipDeviceSynchronize() == 0
+        for fmt, img, w, h, p, d_in, d_out, n in jobs:
+            want = oracle.compress(int(fmt), img, w, h, oracle.params(2, p.weights, False),
+                                   threads=oracle.default_threads())
+            assert np.array_equal(d_out.download(), want)
+    finally:
+        assert hip.hipDeviceSynchronize() == 0
+        for s in streams:
+            hip.hipStreamDestroy(s)

@@ -246,9 +246,9 @@ static hipError_t encode_on(txc::DeviceContext* ctx, const txk::EncodeArgs& a, h
     std::lock_guard<std::mutex> lock(ctx->pipe_mu);
     txp::PipeBufs bufs;
     if (!ctx->pipe_scratch(a.nblocks, &bufs)) return hipErrorOutOfMemory;
-    if (hipStreamWaitEvent(s, ctx->pipe_done, 0) != hipSuccess) return hipErrorUnknown;
+    if (!ctx->pipe_before(s)) return hipErrorUnknown;
     const hipError_t e = txp::launch_pipeline(pipe_launch_for(a, (uint32_t)ctx->cus), bufs, s, marks);
-    if (hipEventRecord(ctx->pipe_done, s) != hipSuccess) return hipErrorUnknown;
+    if (!ctx->pipe_after(s)) return hipErrorUnknown;
     return e;
 }
 

@@ -44,6 +44,7 @@ bool DeviceContext::pipe_scratch(uint64_t recs, txp::PipeBufs* out)
         uint64_t cap = pipe_recs ? pipe_recs : 4096;
         while (cap < recs) cap *= 2;
         if (pipe) {
+            (void)pipe_cover_lib();
             (void)hipEventSynchronize(pipe_done);
             (void)hipFree(pipe);
             pipe = nullptr;
@@ -65,6 +66,32 @@ bool DeviceContext::pipe_scratch(uint64_t recs, txp::PipeBufs* out)
     }
     out->carve(pipe, pipe_recs);
     return true;
+}
+
+bool DeviceContext::pipe_cover_lib()
+{
+    if (!pipe_lib_pending) return true;
+    pipe_lib_pending = false;
+    return hipEventRecord(pipe_done, stream) == hipSuccess;
+}
+
+bool DeviceContext::pipe_before(hipStream_t s)
+{
+    // a use right after one on the same library stream is ordered by it;
+    // otherwise wait for the last recorded use (a never-recorded event is
+    // already complete)
+    if (s == stream && pipe_lib_pending) return true;
+    if (!pipe_cover_lib()) return false;
+    return hipStreamWaitEvent(s, pipe_done, 0) == hipSuccess;
+}
+
+bool DeviceContext::pipe_after(hipStream_t s)
+{
+    if (s == stream) {
+        pipe_lib_pending = true;
+        return true;
+    }
+    return hipEventRecord(pipe_done, s) == hipSuccess;
 }
 
 namespace {
