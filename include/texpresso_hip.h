@@ -301,7 +301,7 @@ int tx_util_time_phases(int format, const uint8_t* d_rgba, size_t width, size_t 
  * BC2/BC3 ClusterFit use 1.25x, BC1 ClusterFit 1.5x and BC1
  * IterativeClusterFit 3x this threshold (the measured crossovers).  Both
  * paths give identical bytes.  0 = always the pipeline, UINT64_MAX = never.
- * Process-wide; returns the previous value (default 45056). */
+ * Process-wide; returns the previous value (default 32768). */
 uint64_t tx_set_pipeline_threshold(uint64_t min_blocks);
 
 /* Human-readable text for an error code. */

@@ -95,7 +95,7 @@ bool DeviceContext::pipe_after(hipStream_t s)
 }
 
 namespace {
-std::atomic<uint64_t> g_pipe_min_blocks{45056};
+std::atomic<uint64_t> g_pipe_min_blocks{32768};
 }
 
 uint64_t pipeline_min_blocks() { return g_pipe_min_blocks.load(); }
