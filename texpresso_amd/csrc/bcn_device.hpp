@@ -663,12 +663,25 @@ __device__ __forceinline__ bool ordering_step(GroupLds& L, int l, int n, bool ac
     group_sync();
     if (unique && l < n) {
         float4 s = L.S[txt::soff(l, l + 1)];
-        for (int t = 0; t < 15; ++t) {
-            const int bb = l + 2 + t;
-            if (bb <= n) {
-                const float4 w = L.S[txt::soff(bb - 1, bb)];
-                s.x = s.x + w.x; s.y = s.y + w.y; s.z = s.z + w.z; s.w = s.w + w.w;
-                L.S[txt::soff(l, bb)] = s;
+        // the diagonal entries this row adds, five at a time ahead of their
+        // dependent adds (one LDS round trip per five instead of per entry;
+        // the loop writes only off-diagonal entries, and reads past the set
+        // are clamped into the table and never used)
+#pragma unroll
+        for (int t0 = 0; t0 < 15; t0 += 5) {
+            float4 w[5];
+#pragma unroll
+            for (int u = 0; u < 5; ++u) {
+                const int bb = min(l + 2 + t0 + u, 16);
+                w[u] = L.S[txt::soff(bb - 1, bb)];
+            }
+#pragma unroll
+            for (int u = 0; u < 5; ++u) {
+                const int bb = l + 2 + t0 + u;
+                if (bb <= n) {
+                    s.x = s.x + w[u].x; s.y = s.y + w[u].y; s.z = s.z + w[u].z; s.w = s.w + w[u].w;
+                    L.S[txt::soff(l, bb)] = s;
+                }
             }
         }
     }
