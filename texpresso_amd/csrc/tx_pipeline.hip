@@ -251,13 +251,19 @@ __global__ __launch_bounds__(256) void k_pipe_bin(const uint32_t* __restrict__ k
     }
 }
 
+// The record fields a finished block's output needs, read with the rest of
+// the record when the group takes it (not behind a load at emit time).
+struct RecOut {
+    uint32_t tex, block, alpha0, alpha1;
+};
+
 // Result of a finished block: the output block, or (NC = 3 for an opaque BC1
 // block) the 3-colour result kept in the record for the 4-colour pass.
 template <int NC>
 __device__ __forceinline__ void emit_block(const BatchTex* __restrict__ tex, const BatchTex& tex0, ClusterRec& R,
-                                           uint32_t info, const GroupLds& L, int l, int n, int remap, float lb_err,
-                                           float thr, uint32_t lb_code, int lb_it, const float (&sa)[3],
-                                           const float (&sb)[3])
+                                           const RecOut& ro, uint32_t info, const GroupLds& L, int l, int n,
+                                           int remap, float lb_err, float thr, uint32_t lb_code, int lb_it,
+                                           const float (&sa)[3], const float (&sb)[3])
 {
     const uint32_t fmt = (info >> 12) & 15u;
     uint32_t lo = 0, hi = 0;
@@ -270,19 +276,19 @@ __device__ __forceinline__ void emit_block(const BatchTex* __restrict__ tex, con
         hi = R.fit3[1];
     }
     if (l == 0) {
-        const BatchTex& T = tex_at(tex, tex0, R.tex);
+        const BatchTex& T = tex_at(tex, tex0, ro.tex);
         if (NC == 3) {
             if ((info >> 8) & 1u) {
-                *reinterpret_cast<uint2*>(T.out + (size_t)R.block * 8u) = make_uint2(lo, hi);
+                *reinterpret_cast<uint2*>(T.out + (size_t)ro.block * 8u) = make_uint2(lo, hi);
             } else {
                 R.fit3_err = won ? lb_err : thr;
                 R.fit3[0] = lo;
                 R.fit3[1] = hi;
             }
         } else if (fmt == BC1) {
-            *reinterpret_cast<uint2*>(T.out + (size_t)R.block * 8u) = make_uint2(lo, hi);
+            *reinterpret_cast<uint2*>(T.out + (size_t)ro.block * 8u) = make_uint2(lo, hi);
         } else {
-            *reinterpret_cast<uint4*>(T.out + (size_t)R.block * 16u) = make_uint4(R.alpha[0], R.alpha[1], lo, hi);
+            *reinterpret_cast<uint4*>(T.out + (size_t)ro.block * 16u) = make_uint4(ro.alpha0, ro.alpha1, lo, hi);
         }
     }
 }
@@ -384,6 +390,7 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
 
     bool idle = false, need = true, ready = false;
     uint32_t r = 0, info = 0, lb_code = 0;
+    RecOut ro{0u, 0u, 0u, 0u};
     int n = 0, iters = 1, it = 0, lb_it = 0, remap = -1;
     float thr = FLT_MAX, lb_err = FLT_MAX;
     float sa[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};
@@ -410,6 +417,7 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
                         const ClusterRec& R = recs[r];
                         L.pts[l] = R.pts[l];
                         info = R.info;
+                        ro = RecOut{R.tex, R.block, R.alpha[0], R.alpha[1]};
                         n = (int)(info & 31u);
                         iters = (int)((info >> 16) & 15u);
                         const uint32_t byte = (R.remap[l >> 2] >> (8 * (l & 3))) & 255u;
@@ -437,7 +445,7 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
             const bool repeated = todo && !unique;
             if (wave_any(repeated)) {
                 if (repeated) {
-                    emit_block<NC>(tex, tex0, recs[r], info, L, l, n, remap, lb_err, thr, lb_code, lb_it, sa, sb);
+                    emit_block<NC>(tex, tex0, recs[r], ro, info, L, l, n, remap, lb_err, thr, lb_code, lb_it, sa, sb);
                     need = true;
                 }
             }
@@ -467,7 +475,7 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
         }
         if (wave_any(fin)) {
             if (fin) {
-                emit_block<NC>(tex, tex0, recs[r], info, L, l, n, remap, lb_err, thr, lb_code, lb_it, sa, sb);
+                emit_block<NC>(tex, tex0, recs[r], ro, info, L, l, n, remap, lb_err, thr, lb_code, lb_it, sa, sb);
                 need = true;
             }
         }
