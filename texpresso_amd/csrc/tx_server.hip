@@ -502,10 +502,10 @@ int tx_server_create(tx_server** out, const tx_server_config* config)
         const size_t n = std::min<size_t>((size_t)16 << 20, s->arena.size());
         std::vector<uint8_t> host(n, 0);
         for (Launch& L : s->launches)
-            if (ok) ok = hipMemcpyAsync(s->arena.base(), host.data(), n, hipMemcpyHostToDevice, L.stream) ==
+            if (ok && n) ok = hipMemcpyAsync(s->arena.base(), host.data(), n, hipMemcpyHostToDevice, L.stream) ==
                              hipSuccess &&
                          hipStreamSynchronize(L.stream) == hipSuccess;
-        if (ok) ok = hipMemcpyAsync(host.data(), s->arena.base(), n, hipMemcpyDeviceToHost, s->fetch_stream) ==
+        if (ok && n) ok = hipMemcpyAsync(host.data(), s->arena.base(), n, hipMemcpyDeviceToHost, s->fetch_stream) ==
                          hipSuccess &&
                      hipStreamSynchronize(s->fetch_stream) == hipSuccess;
     }
