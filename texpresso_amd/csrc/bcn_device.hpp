@@ -817,12 +817,16 @@ __device__ __forceinline__ uint32_t split_index(const GroupLds& L, int n, int re
 // cluster.rs:152-274 / 276-417 with all groups of the wave in lockstep.
 // Executed by the whole wave; `act` marks the groups running ClusterFit.
 // `best_err`, `lo`, `hi` carry ClusterFit::best_error / best_compressed
-// across compress3 -> compress4.
+// across compress3 -> compress4.  `table_ready` (single-ordering searches
+// only): the group's LDS already holds this axis's ordering and prefix
+// table -- compress3 of ClusterFit built them, and its search, endpoints and
+// index assembly only read them -- so compress4 does not build them again.
 template <int NC, int ITERS>
 __device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, int l, bool act, float px, float py,
                                             float pz, const FitParams& fp, float& best_err, uint32_t& lo,
-                                            uint32_t& hi)
+                                            uint32_t& hi, bool table_ready = false)
 {
+    static_assert(ITERS >= 1, "at least one ordering");
     const int n = cs.n;
     const float thr = best_err;
     float lb_err = thr;
@@ -834,7 +838,12 @@ __device__ __forceinline__ void run_cluster(GroupLds& L, const ColourSet& cs, in
 
     for (int it = 0; it < ITERS; ++it) {
         if (!wave_any(!done)) break;
-        if (!ordering_step(L, l, n, !done, it, axx, axy, axz)) done = true;
+        const bool build = !done && !(ITERS == 1 && table_ready);
+        if (wave_any(build)) {
+            // an ordering built at it = 0 is always unique; only later ones can repeat
+            const bool unique = ordering_step(L, l, n, build, it, axx, axy, axz);
+            if (build && !unique) done = true;
+        }
         float be;
         uint32_t bc;
         search_step<NC>(L, l, n, !done, fp, be, bc);
@@ -1195,7 +1204,7 @@ __device__ __forceinline__ void encode_block(uint32_t px, bool valid, const FitP
             uint32_t clo = 0, chi = 0;
             if (FMT == BC1) run_cluster<3, ITERS>(L, cs, l, act, vx, vy, vz, fp, best_err, clo, chi);
             run_cluster<4, ITERS>(L, cs, l, act && !(FMT == BC1 && cs.transparent), vx, vy, vz, fp, best_err, clo,
-                                  chi);
+                                  chi, FMT == BC1 && act);
             if (act) set_colour<FMT>(out, clo, chi);
         }
     }

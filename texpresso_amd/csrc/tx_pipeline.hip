@@ -549,7 +549,8 @@ __global__ __launch_bounds__(64 * kSearchWaves) __attribute__((amdgpu_waves_per_
         float best = FLT_MAX;
         uint32_t lo = 0, hi = 0;  // all-zero block if no candidate wins (colourfit.rs:44-60)
         run_cluster<3, 1>(L, cs, l, has && fmt == BC1, axx, axy, axz, fp, best, lo, hi);
-        run_cluster<4, 1>(L, cs, l, has && !(fmt == BC1 && cs.transparent), axx, axy, axz, fp, best, lo, hi);
+        run_cluster<4, 1>(L, cs, l, has && !(fmt == BC1 && cs.transparent), axx, axy, axz, fp, best, lo, hi,
+                          has && fmt == BC1);
         if (has && l == 0) {
             const BatchTex& T = tex_at(tex, tex0, tex_idx);
             if (fmt == BC1)
