@@ -576,6 +576,9 @@ def run_suite(args):
     if "bc1-512" in which:
         out.append(suite_bc1_512(tx, texgen, O, threads, args.steps, args.warmup))
         print(json.dumps(out[-1]), flush=True)
+    if "bc1-2048" in which:
+        out.append(bc1_cluster_case(tx, texgen, O, 2048, threads, args.steps, args.warmup))
+        print(json.dumps(out[-1]), flush=True)
     if "chains" in which:
         for r in suite_chains(tx, texgen, O, threads, args.steps, args.warmup, args.tuning_table,
                               retune=not args.keep_tuning):
@@ -609,7 +612,7 @@ def main():
                     help="print the plain FP32 VALU issue rate at 1..8 waves per SIMD and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
-    ap.add_argument("--suite", default="", help="comma list of bc1-512,chains,batch,serving: per-workload lines instead of the "
+    ap.add_argument("--suite", default="", help="comma list of bc1-512,bc1-2048,chains,batch,serving: per-workload lines instead of the "
                                                  "default line")
     ap.add_argument("--suite-out", default="", help="write the suite results to this JSON file")
     ap.add_argument("--tuning-table", default=os.path.join(ROOT, "texpresso_amd", "tuning.json"))
