@@ -79,6 +79,20 @@ def test_admission_waits_for_device_memory(gpu, oracle):
         assert np.array_equal(o, oracle.compress(fmt, img, w, h, oracle.params(int(p.algorithm), p.weights, False)))
 
 
+def test_smallest_budget_server(gpu, oracle):
+    # the smallest arena the server accepts (4096 bytes): creation warms the
+    # copy path with copies of the whole arena, and a job that fits still runs
+    srv = tx.Server(hbm_budget=4096, max_inflight=2, max_jobs_per_launch=2)
+    img = texgen.rgb_texture(16, 12, 5)  # 768 bytes in, 96 bytes of BC1 out
+    p = tx.Params(tx.Algorithm.IterativeClusterFit)
+    jid = srv.submit(0, img, 16, 12, p)
+    assert srv.wait(jid, timeout_ms=60000).status == 3
+    got = srv.fetch(jid, tx.Format.Bc1.compressed_size(16, 12))
+    srv.release(jid)
+    srv.close()
+    assert np.array_equal(got, oracle.compress(0, img, 16, 12, oracle.params(2, p.weights, False)))
+
+
 def test_bad_jobs_fail_alone(gpu):
     srv = tx.Server(hbm_budget=4 << 20)
     img = texgen.rgb_texture(64, 64, 3)
